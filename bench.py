@@ -1,0 +1,192 @@
+"""Benchmark: particle-steps/s of the 3D 1M-particle Oldroyd-B step on one MI355X.
+
+Workload: 100^3 lattice, walls normal to y moving at -/+0.25, 16 rotating
+magnetic colloids (R = 0.05), tau = 1, eta = 1, c = 20, np = 100, kT = 1e-3,
+dt = 2e-5, seeded velocity noise sigma = 1e-3 (1,059,xxx particles incl. walls
+and colloid shells).  One step = VV1 + neighbour search + density + velocity
+gradient/dc-dt + forces + wall/colloid forces + VV2, all on the GPU, state
+resident in HBM.  `value` is the FP32 form (pair arithmetic in float,
+positions and reductions in double); the FP64 form is reported beside it.
+
+Prints one JSON line.  --gpus above 1 is not supported (one-GPU build).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "sph-magnetic-particles-oldroyd-b-in-cuda_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+sys.path[:0] = [PKG, ORACLE]
+
+import inputs  # noqa: E402
+import sph_system  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def kernel_bytes(phase, dim, rbytes, store_gradv=False):
+    """Algorithmic (compulsory) HBM bytes per particle for one launch of the
+    phase's main kernel: every array the kernel must read or write, once."""
+    pos = 8 * dim
+    v, c = rbytes * dim, rbytes * (6 if dim == 3 else 3)
+    if phase == "forces":      # x v c dens press type mass -> F
+        return pos + v + c + 2 * rbytes + 4 + rbytes + v
+    if phase == "density":     # x mass -> dens press
+        return pos + rbytes + 2 * rbytes
+    if phase == "gradv_dcdt":  # x v type dens c -> dcdt (+ gradv)
+        return pos + v + 4 + rbytes + c + c + (rbytes * dim * dim if store_gradv else 0)
+    if phase == "vv1":         # type x v F mass c dcdt -> x v c
+        return 4 + pos + v + v + rbytes + c + c + pos + v + c
+    if phase == "vv2":         # type v F mass c dcdt -> v c
+        return 4 + v + v + rbytes + c + c + v + c
+    if phase == "neighbours":  # hash + scatter + rank/gather of x v c mass type id
+        moving = pos + v + c + rbytes + 4 + 4
+        return (pos + 8) + 20 + (8 + 4 + moving + moving + 8)
+    return 0
+
+
+def make_system(precision, n, n_colloids, tmpdir, seed=3):
+    cfg = inputs.walls_colloids_3d(n=n, n_colloids=n_colloids, seed=seed)
+    path = os.path.join(tmpdir, f"input_{precision}")
+    inputs.write_input(path, cfg)
+    s = sph_system.System(precision)
+    s.read_input(path)
+    s.initialize()
+    inputs.seeded_velocity(s, 1e-3, seed)
+    return s, cfg
+
+
+def time_gpu(precision, n, n_colloids, steps, warmup, tmpdir):
+    s, cfg = make_system(precision, n, n_colloids, tmpdir)
+    s.prepare()
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    # Per-phase device time with HIP events on the library's own stream.
+    s.set_timing(True)
+    s.get_timing()  # reset
+    s.step(max(3, min(steps, 10)))
+    s.synchronize()
+    phases = s.get_timing()
+    s.set_timing(False)
+    ke = s.kinetic_energy()
+    N = s.N
+    s.close()
+    return dict(N=N, elapsed=elapsed, phases=phases, ke=ke, cfg=cfg)
+
+
+def cpu_baseline(n, n_colloids, steps, tmpdir):
+    import oracle
+
+    s, _ = make_system(sph_system.FP64, n, n_colloids, tmpdir)
+    o = oracle.Oracle.from_system(s)
+    N = s.N
+    s.close()
+    o.prepare()
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    return dict(value=N * steps / (t1 - t0), cores=oracle.num_threads(), N=N, steps=steps,
+                seconds=t1 - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=100, help="lattice sites per axis")
+    ap.add_argument("--colloids", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    with tempfile.TemporaryDirectory() as tmpdir:
+        r32 = time_gpu(sph_system.FP32, a.n, a.colloids, a.steps, a.warmup, tmpdir)
+        print(f"# fp32: N={r32['N']} {a.steps} steps in {r32['elapsed']:.3f}s", file=sys.stderr,
+              flush=True)
+        r64 = None
+        if not a.no_fp64:
+            r64 = time_gpu(sph_system.FP64, a.n, a.colloids, a.steps, a.warmup, tmpdir)
+            print(f"# fp64: {a.steps} steps in {r64['elapsed']:.3f}s", file=sys.stderr,
+                  flush=True)
+        cpu = None
+        if not a.no_cpu:
+            cpu = cpu_baseline(a.n, a.colloids, a.cpu_steps, tmpdir)
+            print(f"# cpu oracle: {cpu['steps']} steps in {cpu['seconds']:.1f}s on "
+                  f"{cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    N = r32["N"]
+    value = N * a.steps / r32["elapsed"]
+    # dominant kernel by measured device time
+    ph = r32["phases"]
+    dom = max((k for k in ph if ph[k][1] > 0), key=lambda k: ph[k][0])
+    avg_ms = ph[dom][0] / ph[dom][1]
+    bpp = kernel_bytes(dom, 3, 4)
+    achieved = bpp * N / (avg_ms * 1e-3) / 1e9
+    step_ms = 1e3 * r32["elapsed"] / a.steps
+    line = {
+        "metric": "particle-steps/s, 3D 1M-particle Oldroyd-B (FP32 vs FP64)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pairs, f64 positions/reductions",
+        "data": "synthetic (seeded lattice + N(0,1e-3) velocity noise, seeded colloid centres)",
+        "config": {
+            "workload": "3D walls + 16 magnetic colloids, 100^3 lattice",
+            "particles": N,
+            "dim": 3, "L": 1.0, "Vwall": 0.25, "tau": 1.0, "eta": 1.0, "c": 20.0,
+            "np": 100.0, "kT": 1e-3, "coll_R": 0.05, "tau_repulsion": 1.25,
+            "dt": 2e-5, "precision": "fp32-form",
+        },
+        "fp64": None if r64 is None else {
+            "value": N * a.steps / r64["elapsed"],
+            "ms_per_step": 1e3 * r64["elapsed"] / a.steps,
+        },
+        "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in ph.items() if v[1]},
+        "roofline": {
+            "kernel": {"forces": "k_forces", "density": "k_density",
+                       "gradv_dcdt": "k_gradv", "neighbours": "neighbour search",
+                       "vv1": "k_vv1", "vv2": "k_vv2"}.get(dom, dom),
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_particle": bpp,
+            "avg_launch_ms": avg_ms,
+        },
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["value"],
+            "unit": "particle-steps/s",
+            "cores": cpu["cores"],
+            "kind": "port",
+            "sample": f"FP64 OpenMP oracle, same workload (N={cpu['N']}), "
+                      f"{cpu['steps']} steps after the initial force pass",
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

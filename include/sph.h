@@ -1,0 +1,193 @@
+/*
+ * sph.h - C-ABI of the MI355X SPH Oldroyd-B solver (one GPU, gfx950).
+ *
+ * Every entry point replaces one method of the reference's host class
+ * `class_system` (src/class_system.h:146-651) or one step of its driver
+ * (src/main.cu:93-467).  The reference passes (numBlocks, threadsPerBlock)
+ * plus 10-70 raw device pointers to every per-step method; here an opaque
+ * handle owns all host and device state, launch geometry lives inside the
+ * library, and every call returns an int status (SPH_OK == 0).
+ *
+ * Data order: the device keeps particles in cell-sorted order (structure of
+ * arrays).  Every getter/setter and every output file uses particle-ID order,
+ * the order of the reference's arrays, which are never permuted there.
+ *
+ * Threading: one handle is used by one host thread.  Calls that mirror a
+ * reference method are synchronous (results are complete on return), like the
+ * reference's cudaDeviceSynchronize after each launch.  sph_step() may leave
+ * work queued; sph_synchronize() waits for it.
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference: int 0/1 + printf, src/system_read_input.cpp:271-507,
+ *      src/system_initialize.cpp:128-261, device assert src/kernel_cell_ranges.cu:20-23) */
+enum {
+  SPH_OK = 0,
+  SPH_ERR_ARG = 1,          /* bad argument / unknown field name / wrong count      */
+  SPH_ERR_IO = 2,           /* input or output file cannot be opened                */
+  SPH_ERR_PARSE = 3,        /* missing key, dimension mismatch, bad option in input */
+  SPH_ERR_GEOMETRY = 4,     /* cells too big, colloid overlap/outside, N > Nmax     */
+  SPH_ERR_DEVICE = 5,       /* HIP runtime error or no GPU                          */
+  SPH_ERR_OUT_OF_GRID = 6,  /* a particle left the cell grid (reference: assert)    */
+  SPH_ERR_NAN = 7,          /* non-finite kinetic energy                            */
+  SPH_ERR_STATE = 8         /* call out of order (e.g. step before initialize)      */
+};
+
+/* ---- precision forms ---------------------------------------------------------
+ * SPH_FP64: every field and all pair arithmetic in double (the reference built
+ *           with `#define real double`, src/config.h:15-19).
+ * SPH_FP32: positions, time, wall/colloid force-torque and kinetic-energy
+ *           reductions in double; velocities, conformation tensor, density,
+ *           pressure, forces, dc/dt and all pair arithmetic in float. */
+enum { SPH_FP64 = 0, SPH_FP32 = 1 };
+
+/* Physical and geometric parameters.  Replaces the ~52 __constant__ symbols of
+ * src/kernel_declare_constants.cu:11-62 (uploaded at src/main.cu:185-236) and
+ * the derived constants of src/system_initialize.cpp:47-262.  Filled by
+ * sph_initialize(); read with sph_get_params(). */
+typedef struct sph_params {
+  int32_t dim, N, Nmax, wall, Nwalls;
+  int32_t Nxyz[3];                 /* lattice sites per axis, y includes wall layers */
+  int32_t Ncells[3], Ntotal_cells;
+  int32_t N_colloids, Ncells_colloids[3], Ntotal_cells_colloids;
+  int32_t Nlist_walls, Nlist_colloids;
+  int32_t ext_force_type, coll_move, new_sim, Nsteps;
+  int32_t freq_micro, freq_macro, freq_walls, freq_colloids;
+  int32_t pad_;
+  double L[3], cell_size[3], cell_colloids_size[3];
+  double dt, t0, overlap, rho, c, csq, P0, rho0, eta, zeta, a, b;
+  double np, kT, nkT, tau;
+  double rcut, rcutsq, rcut_inv, cw, c_gradw, ky, ext_force[3];
+  double wall_width, y_bottom, y_top, V_top, V_bottom, beta_max;
+  double coll_R, coll_rho, coll_mass, coll_I;
+  double coll_rep_cuton, coll_rep_cutoff, F0_rep, tau_rep;
+  double rcuton_coll_rep, rcuton_coll_rep_sq, rcutoff_coll_rep, rcutoff_coll_rep_sq;
+  double rcutoff_coll, rcutoff_coll_sq;
+  double r0_magnet, r0_magnet_sq, F0_magnet, omega_magnet;
+} sph_params_t;
+
+typedef struct sph_system *sph_handle;
+
+/* Lifecycle.  No reference counterpart: the reference uses a stack object
+ * `class_system sys` (src/main.cu:88) plus loose device pointers freed at
+ * src/main.cu:481-553.  sph_create touches no GPU. */
+sph_handle sph_create(int precision);
+void sph_destroy(sph_handle h);
+const char *sph_last_error(sph_handle h);
+int sph_precision(sph_handle h);
+
+/* class_system::read_input (src/system_read_input.cpp:18-512): same key/value
+ * grammar, every key mandatory, same validation.  Differences: the path is an
+ * argument (reference: fixed "./input"), a missing file is an error
+ * (reference returns 0, line 58-62), the 3D overlap check uses z for z
+ * (reference line 485 uses y). */
+int sph_read_input(sph_handle h, const char *path);
+
+/* class_system::constructor + initialize + initialize_particles_new_sim/old_sim
+ * (src/system_constructor.cpp:15-108, src/system_initialize.cpp:16-359,
+ * src/system_initialize_particles_new_sim.cpp:16-253,
+ * src/system_initialize_particles_old_sim.cpp:17-112).  Host only: device
+ * buffers are allocated by the first device call.  `restart_path` names the
+ * restart file read when new_sim == 1 (reference: fixed "input_particles.in");
+ * NULL means "input_particles.in" next to the input file. */
+int sph_initialize(sph_handle h, const char *restart_path);
+
+int sph_get_params(sph_handle h, sph_params_t *out);
+
+/* Field access in particle-ID order.  Real-valued fields are exchanged as
+ * double (float fields of SPH_FP32 are widened/narrowed), integer fields as
+ * int32.  `count` must equal the field's length.  Names: x y z vx vy vz mass
+ * dens press fx fy fz cxx cxy cyy cxz cyz czz dcdt_xx dcdt_xy dcdt_yy dcdt_xz
+ * dcdt_yz dcdt_zz gradvxx gradvxy gradvxz gradvyx gradvyy gradvyz gradvzx
+ * gradvzy gradvzz type (int, length N); coll_x coll_y coll_z coll_vx coll_vy
+ * coll_vz coll_omegax coll_omegay coll_omegaz coll_theta coll_quat0 coll_quatx
+ * coll_quaty coll_quatz fx_colloid fy_colloid fz_colloid tx_colloid
+ * ty_colloid tz_colloid (length N_colloids); fx_wall fy_wall fz_wall (length
+ * Nwalls); walls_list colloids_list (int, Nlist_*); walls_start colloids_start
+ * (int); x_center y_center z_center (length Nlist_colloids).
+ * Replaces the D2H copies of src/system_print_particles.cpp:42-56 and friends. */
+int sph_get_field(sph_handle h, const char *name, void *dst, int64_t count);
+int sph_set_field(sph_handle h, const char *name, const void *src, int64_t count);
+
+/* Cell list after the last neighbours_search, in the reference's layout
+ * (src/system_neighbours_search.cu:40-67): particle_cell (sorted keys, length N),
+ * particle_index (particle IDs in sorted order, length N), cell_start and
+ * cell_end (length Ntotal_cells; an empty cell has start > end). */
+int sph_get_cell_list(sph_handle h, int32_t *particle_cell, int32_t *particle_index,
+                      int32_t *cell_start, int32_t *cell_end);
+
+/* ---- per-step methods (each synchronous) ---------------------------------- */
+/* class_system::neighbours_search (src/system_neighbours_search.cu:19-110):
+ * cell hash, stable (cell, ID) sort, cell ranges, and the particle state
+ * reordered into cell order on the device. */
+int sph_neighbours_search(sph_handle h);
+/* class_system::calculate_density (src/system_calculate_density.cu:15-49) */
+int sph_calculate_density(sph_handle h);
+/* class_system::calculate_vel_gradient (src/system_calculate_vel_gradient.cu:16-86) */
+int sph_calculate_vel_gradient(sph_handle h);
+/* class_system::calculate_pressure (src/system_calculate_pressure.cu:15-34) */
+int sph_calculate_pressure(sph_handle h);
+/* class_system::calculate_external_forces (src/system_calculate_external_forces.cu:15-40) */
+int sph_calculate_external_forces(sph_handle h);
+/* class_system::calculate_force_walls (src/system_calculate_force_walls.cu:15-38) */
+int sph_calculate_force_walls(sph_handle h);
+/* class_system::calculate_force_colloids (src/system_calculate_force_colloids.cu:15-97) */
+int sph_calculate_force_colloids(sph_handle h);
+/* class_system::calculate_forces (src/system_calculate_forces.cu:16-174): density,
+ * velocity gradient, pressure, pair forces, external force, dc/dt, wall and
+ * colloid force collection, colloid repulsion and magnetic forces. */
+int sph_calculate_forces(sph_handle h);
+/* class_system::V_Verlet_step1 (src/system_V_Verlet_step1.cu:15-122) */
+int sph_V_Verlet_step1(sph_handle h);
+/* class_system::V_Verlet_step2 (src/system_V_Verlet_step2.cu:15-94) */
+int sph_V_Verlet_step2(sph_handle h);
+/* Magnetic moment for time step `step`: m = (cos a, sin a, 0) with
+ * a = omega_magnetic * (t0 + dt * step)  (src/main.cu:290-293, 395-400). */
+int sph_set_step(sph_handle h, int step);
+
+/* ---- drivers -------------------------------------------------------------- */
+/* The initial neighbours_search + calculate_forces before the loop
+ * (src/main.cu:298-335). */
+int sph_prepare(sph_handle h);
+/* `nsteps` iterations of the loop body of src/main.cu:358-459 without output:
+ * VV1, neighbours_search, magnetic moment, calculate_forces, VV2.  Starts at
+ * the handle's step counter + 1 and advances it.  Asynchronous. */
+int sph_step(sph_handle h, int nsteps);
+int sph_synchronize(sph_handle h);
+int sph_current_step(sph_handle h);
+/* The full reference main(): prepare, output at step 0, Nsteps steps with
+ * output at freq_* (src/main.cu:30-561).  Files go to `out_dir`. */
+int sph_run(sph_handle h, const char *out_dir);
+
+/* ---- reductions and output ---------------------------------------------- */
+/* Total kinetic energy sum_i 0.5 m_i |v_i|^2 over all particles, FP64
+ * deterministic reduction (src/kernel_calculate_macro_vars.cu:14-40). */
+int sph_kinetic_energy(sph_handle h, double *ke);
+/* class_system::print_output (src/system_print_output.cpp:14-80): writes
+ * micro-<step>.dat, walls-<step>.dat, colloids-<step>.dat and appends to
+ * macro.dat in `out_dir` when step is a multiple of the matching freq_*. */
+int sph_print_output(sph_handle h, int step, const char *out_dir);
+/* Writes the restart file read by new_sim == 1 (column layout of
+ * src/system_initialize_particles_old_sim.cpp:35-63). */
+int sph_write_restart(sph_handle h, const char *path);
+
+/* ---- instrumentation ----------------------------------------------------- */
+/* When enabled, each device phase is bracketed by HIP events on the handle's
+ * stream.  Phases: 0 VV1, 1 neighbour search, 2 density, 3 velocity gradient
+ * + dc/dt, 4 forces, 5 walls + colloids, 6 VV2.  `ms` receives the summed
+ * milliseconds per phase since the last reset and `calls` the launch counts. */
+int sph_set_timing(sph_handle h, int enable);
+int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_AMD_H */

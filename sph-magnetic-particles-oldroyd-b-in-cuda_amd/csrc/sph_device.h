@@ -1,0 +1,52 @@
+// sph_device.h - interface between the C-ABI layer and the gfx950 device code.
+//
+// One Device object owns every device buffer of one simulation: particle state
+// in cell-sorted SoA order (double-buffered for the reorder), the cell list,
+// the inverse permutation ID -> sorted position, colloid rigid-body state and
+// reduction scratch.  All launches go to one HIP stream owned by the object.
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "sph_host.h"
+
+namespace sph {
+
+enum Phase { PH_VV1, PH_NS, PH_DENS, PH_GRADV, PH_FORCES, PH_BODIES, PH_VV2, PH_OTHER, PH_NUM };
+
+class Device {
+ public:
+  virtual ~Device() = default;
+  // Allocates and uploads the host state (ID order becomes the initial order).
+  virtual int upload(const HostState &hs, std::string &err) = 0;
+  virtual int neighbours_search(std::string &err) = 0;
+  virtual int density(std::string &err) = 0;
+  virtual int pressure(std::string &err) = 0;
+  virtual int vel_gradient(bool store_gradv, std::string &err) = 0;
+  virtual int pair_forces(bool with_ext, std::string &err) = 0;
+  virtual int external_forces(std::string &err) = 0;
+  virtual int force_walls(std::string &err) = 0;
+  virtual int force_colloids(std::string &err) = 0;
+  virtual int vv1(std::string &err) = 0;
+  virtual int vv2(std::string &err) = 0;
+  virtual void set_magnetic_moment(const double m[3]) = 0;
+  virtual int kinetic_energy(double *ke, std::string &err) = 0;
+  virtual int synchronize(std::string &err) = 0;
+  // Checks the device error flag (out-of-grid particle); resets it.
+  virtual int check_flags(std::string &err) = 0;
+
+  // Copies a field into `hs` (ID order).  `name` as in sph.h.
+  virtual int download_field(HostState &hs, const std::string &name, std::string &err) = 0;
+  virtual int upload_field(const HostState &hs, const std::string &name, std::string &err) = 0;
+  virtual int cell_list(int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce,
+                        std::string &err) = 0;
+
+  virtual void set_timing(bool on) = 0;
+  virtual int get_timing(double ms[8], int64_t calls[8], std::string &err) = 0;
+};
+
+// precision: SPH_FP64 or SPH_FP32
+std::unique_ptr<Device> make_device(int precision);
+
+}  // namespace sph

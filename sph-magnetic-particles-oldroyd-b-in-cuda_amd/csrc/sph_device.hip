@@ -1,0 +1,1521 @@
+// sph_device.hip - gfx950 kernels and the device-side owner of one simulation.
+//
+// Layout: particles live in cell-sorted order as structure-of-arrays.  Positions
+// are double in both precision forms; every other per-particle field is `R`
+// (double for SPH_FP64, float for SPH_FP32).  Each neighbours_search hashes
+// the particles, counts per cell, scans the counts into cell_start, and moves
+// every particle to cell_start[cell] + (number of particles of its cell with a
+// smaller ID) - the stable (cell, ID) order of the reference's Thrust sort - while
+// gathering its state into the second buffer set.  Neighbour cells x-1..x+1 of
+// one (y, z) row are then one contiguous range ("strip") of the arrays, so the
+// pair passes read neighbours coalesced instead of through particle_index.
+//
+// Kernels and the reference kernels they replace:
+//   k_hash, k_scan_cells, k_scatter, k_rank_gather   kernel_cell_list.cu:15-63,
+//       thrust::sort_by_key (system_neighbours_search.cu:56-58), kernel_cell_ranges.cu:13-33
+//   k_density            kernel_densities.cu:16-193 + kernel_pressures.cu:13-23
+//   k_gradv              kernel_vel_gradients.cu:15-662 + kernel_dcdt.cu:15-104
+//   k_forces             kernel_forces.cu:15-865 + kernel_ext_force.cu:13-73
+//   k_walls              kernel_collect_force_walls.cu:15-50 (wavefront reduction)
+//   k_colloids           kernel_collect_force_colloids.cu:15-99,
+//                        kernel_repulsion_force_colloids.cu:21-207,
+//                        kernel_magnetic_force_colloids.cu:15-203
+//   k_vv1, k_vv2         kernel_move_sys_VV_part1.cu:13-114, kernel_move_sys_VV_part2.cu:13-58
+//   k_coll_vv1, k_coll_place, k_coll_vv2
+//                        kernel_move_colloids_VV_part1.cu:15-204, kernel_move_colloids_VV_part2.cu:14-47
+//   k_ke_partial, k_ke_final  kernel_calculate_macro_vars.cu:14-40 (deterministic FP64)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "sph_device.h"
+
+namespace sph {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlock = 1024;
+constexpr int kKeBlocks = 1024;
+
+#define HIPCHK(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+      return SPH_ERR_DEVICE;                                                  \
+    }                                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device views
+// ---------------------------------------------------------------------------
+template <class R>
+struct Particles {  // per-particle state that moves with the reorder
+  double *x, *y, *z;
+  R *v[3];
+  R *c[6];  // xx xy yy xz yz zz
+  R *mass;
+  int *type, *id;
+};
+
+template <class R>
+struct Derived {  // per-particle fields recomputed every step
+  R *dens, *press;
+  R *f[3];
+  R *dcdt[6];
+  R *gv[9];  // xx xy xz yx yy yz zx zy zz ; gv_ab = dv_b/dx_a
+};
+
+struct Bodies {  // colloids (double) and wall/colloid lists (particle IDs)
+  double *s[C_NUM];
+  const int *clist, *cstart, *cowner;
+  const double *xc, *yc, *zc;
+  const int *wlist, *wstart;
+  double *fwall;  // [3][2]
+  int n_colloids;
+};
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mimg(double d, double L, double h) {
+  if (d > h) d -= L;
+  if (d < -h) d += L;
+  return d;
+}
+
+// kernel_W (src/kernel_W.cu:13-59) as selects: cw*((a^5 - 6 b^5) + 15 c^5).
+template <class R>
+__device__ __forceinline__ R sph_W(R r, R rcut_inv, R cw) {
+  R q3 = r * rcut_inv * R(3);
+  R a = R(3) - q3, a2 = a * a, t = a2 * a2 * a;
+  R b = R(2) - q3, b2 = b * b, b5 = R(6) * (b2 * b2 * b);
+  R c = R(1) - q3, c2 = c * c, c5 = R(15) * (c2 * c2 * c);
+  if (q3 < R(2)) t = t - b5;
+  if (q3 < R(1)) t = t + c5;
+  return cw * t;
+}
+
+// kernel_gradW (src/kernel_gradW.cu:13-53): c_gradw*((a^4 - 6 b^4) + 15 c^4).
+template <class R>
+__device__ __forceinline__ R sph_gradW(R r, R rcut_inv, R c_gradw) {
+  R q3 = R(3) * (r * rcut_inv);
+  R a = R(3) - q3, a2 = a * a, t = a2 * a2;
+  R b = R(2) - q3, b2 = b * b, b4 = R(6) * (b2 * b2);
+  R c = R(1) - q3, c2 = c * c, c4 = R(15) * (c2 * c2);
+  if (q3 < R(2)) t = t - b4;
+  if (q3 < R(1)) t = t + c4;
+  return c_gradw * t;
+}
+
+// Unclamped cell coordinates of a particle, as the pair kernels of the
+// reference compute them (e.g. src/kernel_densities.cu:47-53).
+template <int DIM, bool WALL>
+__device__ __forceinline__ void cell_coords(const sph_params_t &P, double x, double y, double z,
+                                            int &cx, int &cy, int &cz) {
+  cx = (int)floor(x / P.cell_size[0]);
+  cy = WALL ? (int)floor((y + P.wall_width) / P.cell_size[1]) : (int)floor(y / P.cell_size[1]);
+  cz = DIM == 3 ? (int)floor(z / P.cell_size[2]) : 0;
+}
+
+// Calls fn(k0, k1) for each contiguous sorted range covering the 9 (2D: 3)
+// neighbour strips of cell (cx, cy, cz).  Interior strips are one range of
+// three consecutive cells; at the periodic x seam the three cells are visited
+// one by one.  Rows outside the wall-bounded y range are skipped.
+template <int DIM, bool WALL, class Fn>
+__device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int *__restrict__ cs,
+                                               int cx, int cy, int cz, Fn &&fn) {
+  const int Nx = P.Ncells[0], Ny = P.Ncells[1], Nz = P.Ncells[2];
+#pragma unroll
+  for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+    int z = cz + dz;
+    if (DIM == 3) {
+      if (z < 0) z += Nz;
+      if (z >= Nz) z -= Nz;
+    }
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      int y = cy + dy;
+      if (!WALL) {
+        if (y < 0) y += Ny;
+        if (y >= Ny) y -= Ny;
+      } else if (y < 0 || y >= Ny) {
+        continue;
+      }
+      const int base = (z * Ny + y) * Nx;
+      if (cx >= 1 && cx <= Nx - 2) {
+        fn(cs[base + cx - 1], cs[base + cx + 2]);
+      } else {
+        for (int dx = -1; dx <= 1; ++dx) {
+          int x = cx + dx;
+          if (x < 0) x += Nx;
+          if (x >= Nx) x -= Nx;
+          fn(cs[base + x], cs[base + x + 1]);
+        }
+      }
+    }
+  }
+}
+
+// r_ij = x_i - x_j with the minimum image (x, z, and y without walls), computed
+// in double and rounded to R.
+template <class R, int DIM, bool WALL>
+__device__ __forceinline__ void pair_sep(const sph_params_t &P, double xi, double yi, double zi,
+                                         double xj, double yj, double zj, R r[3]) {
+  r[0] = (R)mimg(xi - xj, P.L[0], 0.5 * P.L[0]);
+  r[1] = WALL ? (R)(yi - yj) : (R)mimg(yi - yj, P.L[1], 0.5 * P.L[1]);
+  r[2] = DIM == 3 ? (R)mimg(zi - zj, P.L[2], 0.5 * P.L[2]) : R(0);
+}
+
+// Relative velocity v_ij with Morris boundary conditions for wall and colloid
+// partners (src/kernel_vel_gradients.cu:157-332, 415-630; the same branches in
+// src/kernel_forces.cu:196-419, 533-822).  Called only when ti == tj == 0 or
+// ti != tj.
+template <class R, int DIM, bool WALL>
+__device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &B, int ti, int tj,
+                                           double xi, double yi, double zi, double xj, double yj,
+                                           double zj, const R vi[3], const R vj[3],
+                                           const R rij[3], R v[3]) {
+  const R bmax = (R)P.beta_max;
+  if (ti == tj) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = vi[a] - vj[a];
+    return;
+  }
+  if (ti == 0) {
+    if (tj == 1 || tj == 2) {
+      R di, dj, Vw;
+      if (tj == 1) {
+        di = (R)(yi - P.y_bottom); dj = (R)(P.y_bottom - yj); Vw = (R)P.V_bottom;
+      } else {
+        di = (R)(P.y_top - yi); dj = (R)(yj - P.y_top); Vw = (R)P.V_top;
+      }
+      R beta = R(1) + dj / di;
+      if (beta > bmax) beta = bmax;
+      if (di > R(0)) {
+        v[0] = beta * (vi[0] - Vw); v[1] = beta * vi[1]; v[2] = beta * vi[2];
+      } else {
+        v[0] = vi[0] - Vw; v[1] = vi[1]; v[2] = vi[2];
+      }
+    } else {
+      const int c = tj - 3;
+      R rc[3];
+      pair_sep<R, DIM, WALL>(P, xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
+                             DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
+      R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
+      const R Rc = (R)P.coll_R;
+      R dis = dc - Rc;
+      const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
+      if (dis < R(0)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = vi[a] - Vc[a];
+      } else {
+        R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
+        R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
+        R ox = DIM == 3 ? (R)B.s[C_OX][c] : R(0), oy = DIM == 3 ? (R)B.s[C_OY][c] : R(0);
+        R oz = (R)B.s[C_OZ][c];
+        R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
+        R djs = (rij[0] * sv[0] + rij[1] * sv[1] + rij[2] * sv[2]) - dis;
+        R beta = R(1) + djs / dis;
+        if (beta > bmax) beta = bmax;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = beta * (vi[a] - Vc[a] - vr[a]);
+      }
+    }
+  } else {
+    if (ti == 1 || ti == 2) {
+      R di, dj, Vw;
+      if (ti == 1) {
+        di = (R)(P.y_bottom - yi); dj = (R)(yj - P.y_bottom); Vw = (R)P.V_bottom;
+      } else {
+        di = (R)(yi - P.y_top); dj = (R)(P.y_top - yj); Vw = (R)P.V_top;
+      }
+      R beta = R(1) + di / dj;
+      if (beta > bmax) beta = bmax;
+      if (dj > R(0)) {
+        v[0] = beta * (Vw - vj[0]); v[1] = -beta * vj[1]; v[2] = -beta * vj[2];
+      } else {
+        v[0] = Vw - vj[0]; v[1] = -vj[1]; v[2] = -vj[2];
+      }
+    } else {
+      const int c = ti - 3;
+      R rc[3];
+      pair_sep<R, DIM, WALL>(P, xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
+                             DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
+      R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
+      const R Rc = (R)P.coll_R;
+      R djs = dc - Rc;
+      const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
+      if (djs < R(0)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = Vc[a] - vj[a];
+      } else {
+        R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
+        R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
+        R ox = DIM == 3 ? (R)B.s[C_OX][c] : R(0), oy = DIM == 3 ? (R)B.s[C_OY][c] : R(0);
+        R oz = (R)B.s[C_OZ][c];
+        R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
+        R dis = (-rij[0] * sv[0] - rij[1] * sv[1] - rij[2] * sv[2]) - djs;
+        R beta = R(1) + dis / djs;
+        if (beta > bmax) beta = bmax;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = beta * (Vc[a] + vr[a] - vj[a]);
+      }
+    }
+  }
+  if (DIM == 2) v[2] = R(0);
+}
+
+// Block-wide sum of `n` doubles per thread; result valid in thread 0.
+template <int NV, int BS>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[BS / 64][NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double t = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    v[k] = t;
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wid][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+      for (int w = 0; w < BS / 64; ++w) t += red[w][k];
+      v[k] = t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour search: hash, count, scan, scatter, stable rank + reorder
+// ---------------------------------------------------------------------------
+// kernel_cell_list (src/kernel_cell_list.cu:15-63): clamp c == Ncells for x, z
+// and for y without walls; out-of-grid cells raise the device flag.
+__global__ __launch_bounds__(kBlock) void k_hash(sph_params_t P, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z, int *cell, int *slot,
+                                                 int *count, int *flag, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int cx = (int)floor(x[i] / P.cell_size[0]);
+  if (cx == P.Ncells[0]) cx = cx - 1;
+  int cy;
+  if (P.wall == 0) {
+    cy = (int)floor(y[i] / P.cell_size[1]);
+    if (cy == P.Ncells[1]) cy = cy - 1;
+  } else {
+    cy = (int)floor((y[i] + P.wall_width) / P.cell_size[1]);
+  }
+  int c;
+  if (P.dim == 3) {
+    int cz = (int)floor(z[i] / P.cell_size[2]);
+    if (cz == P.Ncells[2]) cz = cz - 1;
+    bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] && cz >= 0 &&
+              cz < P.Ncells[2];
+    c = ok ? cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
+  } else {
+    bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1];
+    c = ok ? cy * P.Ncells[0] + cx : -1;
+  }
+  if (c < 0) {
+    atomicOr(flag, 1);
+    cell[i] = -1;
+    return;
+  }
+  cell[i] = c;
+  slot[i] = atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of the per-cell counts into cs[0..ncells]; zeroes the counts
+// for the next hash.  One workgroup of 1024 threads, each owning a contiguous
+// chunk of cells.
+__global__ __launch_bounds__(1024) void k_scan_cells(int *count, int *cs, int ncells) {
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int chunk = (ncells + 1023) / 1024;
+  const int c0 = min(t * chunk, ncells), c1 = min(c0 + chunk, ncells);
+  int local = 0;
+  for (int c = c0; c < c1; ++c) local += count[c];
+  // inclusive scan of `local` across the block
+  int incl = local;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int n = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += n;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int w = 0; w < 16; ++w) {
+      int s = wsum[w];
+      wsum[w] = acc;
+      acc += s;
+    }
+  }
+  __syncthreads();
+  int run = wsum[wid] + incl - local;
+  for (int c = c0; c < c1; ++c) {
+    int n = count[c];
+    cs[c] = run;
+    run += n;
+    count[c] = 0;
+  }
+  if (t == 1023) cs[ncells] = run;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const int *__restrict__ cell,
+                                                    const int *__restrict__ slot,
+                                                    const int *__restrict__ cs,
+                                                    const int *__restrict__ id, int *stage_src,
+                                                    int *stage_id, int N) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  int c = cell[k];
+  if (c < 0) return;
+  int p = cs[c] + slot[k];
+  stage_src[p] = k;
+  stage_id[p] = id[k];
+}
+
+// Final position = cell start + number of particles of the same cell with a
+// smaller ID, i.e. the stable (cell, ID) order.  Gathers the moving state.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_rank_gather(
+    Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
+    const int *__restrict__ cs, const int *__restrict__ stage_src,
+    const int *__restrict__ stage_id, int *rank, int *sorted_cell, int ncells, int nc, int N) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N || p >= cs[ncells]) return;
+  const int k = stage_src[p];
+  const int c = cell[k];
+  const int s0 = cs[c], s1 = cs[c + 1];
+  const int myid = stage_id[p];
+  int r = 0;
+  for (int q = s0; q < s1; ++q) r += stage_id[q] < myid;
+  const int d = s0 + r;
+  dst.x[d] = src.x[k];
+  dst.y[d] = src.y[k];
+  dst.z[d] = src.z[k];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) dst.v[a][d] = src.v[a][k];
+  for (int a = 0; a < nc; ++a) dst.c[a][d] = src.c[a][k];
+  dst.mass[d] = src.mass[k];
+  dst.type[d] = src.type[k];
+  dst.id[d] = myid;
+  rank[myid] = d;
+  sorted_cell[d] = c;
+}
+
+// ---------------------------------------------------------------------------
+// pair pass 1: density (+ pressure)
+// ---------------------------------------------------------------------------
+template <class R, int DIM, bool WALL>
+__global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                    const int *__restrict__ cs, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *__restrict__ X = S.x;
+  const double *__restrict__ Y = S.y;
+  const double *__restrict__ Z = S.z;
+  const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
+  int cx, cy, cz;
+  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
+  const R rcutsq = (R)P.rcutsq, rinv = (R)P.rcut_inv, cw = (R)P.cw;
+  R d = R(0);
+  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+    for (int k = k0; k < k1; ++k) {
+      R r[3];
+      pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
+      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+      if (r2 < rcutsq) d = d + sph_W<R>(sqrt(r2), rinv, cw);
+    }
+  });
+  D.dens[i] = d;
+  D.press[i] = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pressure(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                     int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  D.press[i] = (R)P.csq * (S.mass[i] * D.dens[i] - (R)P.rho0);
+}
+
+// ---------------------------------------------------------------------------
+// pair pass 2: velocity gradient (+ dc/dt of fluid particles)
+// ---------------------------------------------------------------------------
+template <class R, int DIM, bool WALL>
+__global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                  Bodies B, const int *__restrict__ cs, int N,
+                                                  int store_gv, int write_dcdt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *__restrict__ X = S.x;
+  const double *__restrict__ Y = S.y;
+  const double *__restrict__ Z = S.z;
+  const int *__restrict__ T = S.type;
+  const R *__restrict__ VX = S.v[0];
+  const R *__restrict__ VY = S.v[1];
+  const R *__restrict__ VZ = S.v[2];
+  const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
+  const int ti = T[i];
+  const R vi[3] = {VX[i], VY[i], DIM == 3 ? VZ[i] : R(0)};
+  const R dinv = R(1) / D.dens[i];
+  int cx, cy, cz;
+  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
+  const R rcutsq = (R)P.rcutsq, rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
+  R g[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) g[a][b] = R(0);
+  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+    for (int k = k0; k < k1; ++k) {
+      if (k == i) continue;
+      const int tj = T[k];
+      if (ti == tj && ti != 0) continue;
+      const double xj = X[k], yj = Y[k], zj = DIM == 3 ? Z[k] : 0.0;
+      R r[3];
+      pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
+      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+      if (!(r2 < rcutsq)) continue;
+      R rr = sqrt(r2), ri = R(1) / rr;
+      R e[3] = {r[0] * ri, r[1] * ri, r[2] * ri};
+      R gw = sph_gradW<R>(rr, rinv_c, cg);
+      const R vj[3] = {VX[k], VY[k], DIM == 3 ? VZ[k] : R(0)};
+      R v[3];
+      morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
+      R coef = -dinv * gw;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        R ce = coef * e[a];
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) g[a][b] += ce * v[b];
+      }
+    }
+  });
+  if (store_gv) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = g[a][b];
+  }
+  if (!write_dcdt) return;
+  // kernel_dcdt (src/kernel_dcdt.cu:15-104); zero for non-fluid particles.
+  if (ti != 0) {
+#pragma unroll
+    for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) D.dcdt[k][i] = R(0);
+    return;
+  }
+  const R tinv = R(1) / (R)P.tau;
+  const R cxx = S.c[0][i], cxy = S.c[1][i], cyy = S.c[2][i];
+  if (DIM == 2) {
+    R cgxx = cxx * g[0][0] + cxy * g[1][0], cgxy = cxx * g[0][1] + cxy * g[1][1];
+    R cgyx = cxy * g[0][0] + cyy * g[1][0], cgyy = cxy * g[0][1] + cyy * g[1][1];
+    D.dcdt[0][i] = R(2) * cgxx + tinv * (R(1) - cxx);
+    D.dcdt[1][i] = cgxy + cgyx - tinv * cxy;
+    D.dcdt[2][i] = R(2) * cgyy + tinv * (R(1) - cyy);
+  } else {
+    const R cxz = S.c[3][i], cyz = S.c[4][i], czz = S.c[5][i];
+    R cgxx = cxx * g[0][0] + cxy * g[1][0] + cxz * g[2][0];
+    R cgxy = cxx * g[0][1] + cxy * g[1][1] + cxz * g[2][1];
+    R cgxz = cxx * g[0][2] + cxy * g[1][2] + cxz * g[2][2];
+    R cgyx = cxy * g[0][0] + cyy * g[1][0] + cyz * g[2][0];
+    R cgyy = cxy * g[0][1] + cyy * g[1][1] + cyz * g[2][1];
+    R cgyz = cxy * g[0][2] + cyy * g[1][2] + cyz * g[2][2];
+    R cgzx = cxz * g[0][0] + cyz * g[1][0] + czz * g[2][0];
+    R cgzy = cxz * g[0][1] + cyz * g[1][1] + czz * g[2][1];
+    R cgzz = cxz * g[0][2] + cyz * g[1][2] + czz * g[2][2];
+    D.dcdt[0][i] = R(2) * cgxx + tinv * (R(1) - cxx);
+    D.dcdt[1][i] = cgxy + cgyx - tinv * cxy;
+    D.dcdt[2][i] = R(2) * cgyy + tinv * (R(1) - cyy);
+    D.dcdt[3][i] = cgxz + cgzx - tinv * cxz;
+    D.dcdt[4][i] = cgyz + cgzy - tinv * cyz;
+    D.dcdt[5][i] = R(2) * cgzz + tinv * (R(1) - czz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pair pass 3: forces (+ external force)
+// ---------------------------------------------------------------------------
+template <class R, int DIM>
+__device__ __forceinline__ void pi_over_d2(const R *const c[6], int k, R nkT, R pr, R dinv2,
+                                           R out[6]) {
+  out[0] = (pr - nkT * c[0][k]) * dinv2;
+  out[1] = (-nkT * c[1][k]) * dinv2;
+  out[2] = (pr - nkT * c[2][k]) * dinv2;
+  if (DIM == 3) {
+    out[3] = (-nkT * c[3][k]) * dinv2;
+    out[4] = (-nkT * c[4][k]) * dinv2;
+    out[5] = (pr - nkT * c[5][k]) * dinv2;
+  } else {
+    out[3] = out[4] = out[5] = R(0);
+  }
+}
+
+template <class R, int DIM>
+__device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, R f[3]) {
+  // kernel_ext_force (src/kernel_ext_force.cu:22-72), fluid particles only.
+  if (P.ext_force_type == 1) {
+    f[0] += m * (R)P.ext_force[0];
+    f[1] += m * (R)P.ext_force[1];
+    if (DIM == 3) f[2] += m * (R)P.ext_force[2];
+  } else if (P.ext_force_type == 2) {
+    f[0] += m * (R)P.ext_force[0] * (R)sin(P.ky * y);
+  } else if (P.ext_force_type == 3) {
+    if (y < 0.5 * P.L[1]) f[0] += m * (R)P.ext_force[0];
+    else f[0] += -(m * (R)P.ext_force[0]);
+  }
+}
+
+template <class R, int DIM, bool WALL>
+__global__ __launch_bounds__(kBlock) void k_forces(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                   Bodies B, const int *__restrict__ cs, int N,
+                                                   int with_ext) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *__restrict__ X = S.x;
+  const double *__restrict__ Y = S.y;
+  const double *__restrict__ Z = S.z;
+  const int *__restrict__ T = S.type;
+  const R *__restrict__ VX = S.v[0];
+  const R *__restrict__ VY = S.v[1];
+  const R *__restrict__ VZ = S.v[2];
+  const R *__restrict__ DENS = D.dens;
+  const R *__restrict__ PRESS = D.press;
+  const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
+  const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
+  const int ti = T[i];
+  const R vi[3] = {VX[i], VY[i], DIM == 3 ? VZ[i] : R(0)};
+  const R nkT = (R)P.nkT;
+  const R di0inv = R(1) / DENS[i];
+  const R pi0 = PRESS[i];
+  R Pi0[6];
+  pi_over_d2<R, DIM>(C6, i, nkT, pi0, di0inv * di0inv, Pi0);
+  int cx, cy, cz;
+  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
+  const R rcutsq = (R)P.rcutsq, rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
+  const R ca = (R)P.a, cb = (R)P.b;
+  R f[3] = {R(0), R(0), R(0)};
+  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+    for (int k = k0; k < k1; ++k) {
+      if (k == i) continue;
+      const int tj = T[k];
+      if (ti != 0 && tj != 0) continue;
+      const double xj = X[k], yj = Y[k], zj = DIM == 3 ? Z[k] : 0.0;
+      R r[3];
+      pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
+      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+      if (!(r2 < rcutsq)) continue;
+      R Pi[6], Pj[6], diinv, djinv;
+      if (ti == tj) {
+        diinv = di0inv;
+        djinv = R(1) / DENS[k];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
+        pi_over_d2<R, DIM>(C6, k, nkT, PRESS[k], djinv * djinv, Pj);
+      } else if (ti == 0) {
+        diinv = di0inv;
+        djinv = di0inv;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
+        pi_over_d2<R, DIM>(C6, k, nkT, pi0, djinv * djinv, Pj);
+      } else {
+        djinv = R(1) / DENS[k];
+        diinv = djinv;
+        pi_over_d2<R, DIM>(C6, k, nkT, PRESS[k], djinv * djinv, Pj);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
+      }
+      const R vj[3] = {VX[k], VY[k], DIM == 3 ? VZ[k] : R(0)};
+      R v[3];
+      morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
+      R rr = sqrt(r2), ri = R(1) / rr;
+      R gw = sph_gradW<R>(rr, rinv_c, cg);
+      R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
+      R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
+      R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]), ayy = -gw * (Pi[2] + Pj[2]);
+      if (DIM == 3) {
+        R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]), azz = -gw * (Pi[5] + Pj[5]);
+        f[0] += axx * e[0] + axy * e[1] + axz * e[2];
+        f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
+        f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
+      } else {
+        f[0] += axx * e[0] + axy * e[1];
+        f[1] += axy * e[0] + ayy * e[1];
+      }
+      R aux = gw * diinv * djinv * ri;
+      R a1 = ca * aux, a2 = cb * aux * ev;
+      f[0] += a1 * v[0] + a2 * e[0];
+      f[1] += a1 * v[1] + a2 * e[1];
+      if (DIM == 3) f[2] += a1 * v[2] + a2 * e[2];
+    }
+  });
+  if (with_ext && ti == 0) ext_force<R, DIM>(P, S.mass[i], yi, f);
+  D.f[0][i] = f[0];
+  D.f[1][i] = f[1];
+  D.f[2][i] = DIM == 3 ? f[2] : R(0);
+}
+
+template <class R, int DIM>
+__global__ __launch_bounds__(kBlock) void k_ext(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || S.type[i] != 0) return;
+  R f[3] = {D.f[0][i], D.f[1][i], DIM == 3 ? D.f[2][i] : R(0)};
+  ext_force<R, DIM>(P, S.mass[i], S.y[i], f);
+  D.f[0][i] = f[0];
+  D.f[1][i] = f[1];
+  if (DIM == 3) D.f[2][i] = f[2];
+}
+
+// ---------------------------------------------------------------------------
+// wall and colloid force collection (FP64, deterministic block reductions)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kRedBlock) void k_walls(sph_params_t P, Derived<R> D, Bodies B,
+                                                     const int *__restrict__ rank) {
+  const int w = blockIdx.x;
+  const int j0 = B.wstart[w];
+  const int j1 = w < P.Nwalls - 1 ? B.wstart[w + 1] : P.Nlist_walls;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int j = j0 + threadIdx.x; j < j1; j += kRedBlock) {
+    int p = rank[B.wlist[j]];
+    acc[0] += (double)D.f[0][p];
+    acc[1] += (double)D.f[1][p];
+    acc[2] += (double)D.f[2][p];
+  }
+  block_sum<3, kRedBlock>(acc);
+  if (threadIdx.x == 0) {
+    B.fwall[0 * 2 + w] = acc[0];
+    B.fwall[1 * 2 + w] = acc[1];
+    B.fwall[2 * 2 + w] = P.dim == 3 ? acc[2] : 0.0;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_colloids(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                     Bodies B, const int *__restrict__ rank,
+                                                     double m0, double m1, double m2) {
+  const int ci = blockIdx.x;
+  const bool d3 = P.dim == 3;
+  const int j0 = B.cstart[ci], j1 = B.cstart[ci + 1];
+  const double Xc = B.s[C_X][ci], Yc = B.s[C_Y][ci], Zc = B.s[C_Z][ci];
+  double acc[6] = {0, 0, 0, 0, 0, 0};  // fx fy fz tx ty tz
+  for (int j = j0 + threadIdx.x; j < j1; j += kBlock) {
+    int p = rank[B.clist[j]];
+    double Fx = D.f[0][p], Fy = D.f[1][p], Fz = d3 ? (double)D.f[2][p] : 0.0;
+    double rx = mimg(S.x[p] - Xc, P.L[0], 0.5 * P.L[0]);
+    double ry = S.y[p] - Yc;
+    if (P.wall == 0) ry = mimg(ry, P.L[1], 0.5 * P.L[1]);
+    double rz = d3 ? mimg(S.z[p] - Zc, P.L[2], 0.5 * P.L[2]) : 0.0;
+    acc[0] += Fx;
+    acc[1] += Fy;
+    acc[2] += Fz;
+    acc[3] += ry * Fz - rz * Fy;
+    acc[4] += rz * Fx - rx * Fz;
+    acc[5] += rx * Fy - ry * Fx;
+  }
+  block_sum<6, kBlock>(acc);
+  if (threadIdx.x != 0) return;
+  double F[3] = {acc[0], acc[1], acc[2]};
+  if (B.n_colloids >= 2) {
+    const double mag[3] = {m0, m1, m2};
+    // Repulsion, then magnetic force, over all other colloids within their
+    // cut-offs (the set the reference finds through its colloid cell list).
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int j = 0; j < B.n_colloids; ++j) {
+        if (j == ci) continue;
+        double r[3];
+        r[0] = mimg(Xc - B.s[C_X][j], P.L[0], 0.5 * P.L[0]);
+        r[1] = Yc - B.s[C_Y][j];
+        if (P.wall == 0) r[1] = mimg(r[1], P.L[1], 0.5 * P.L[1]);
+        r[2] = d3 ? mimg(Zc - B.s[C_Z][j], P.L[2], 0.5 * P.L[2]) : 0.0;
+        double r2 = d3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+        if (pass == 0) {
+          if (r2 < P.rcuton_coll_rep_sq) r2 = P.rcuton_coll_rep_sq;
+          if (r2 <= P.rcutoff_coll_rep_sq) {
+            double rr = sqrt(r2);
+            double ts = -P.tau_rep * (rr - 2.0 * P.coll_R) / P.coll_R;
+            double ex = exp(ts);
+            double Fm = (P.F0_rep * P.tau_rep) * ex / (1.0 - ex);
+            for (int a = 0; a < P.dim; ++a) F[a] = F[a] + Fm * (r[a] / rr);
+          }
+        } else if (r2 <= P.r0_magnet_sq) {
+          double rr = d3 ? sqrt(r2) / P.coll_R : sqrt(r2);
+          double e[3] = {r[0] / rr, r[1] / rr, d3 ? r[2] / rr : 0.0};
+          double me = d3 ? mag[0] * e[0] + mag[1] * e[1] + mag[2] * e[2]
+                         : mag[0] * e[0] + mag[1] * e[1];
+          double me2 = me * me;
+          double F0r4 = P.F0_magnet / (r2 * r2);
+          for (int a = 0; a < P.dim; ++a)
+            F[a] = F[a] + F0r4 * (2.0 * me * mag[a] - (5.0 * me2 - 1.0) * e[a]);
+        }
+      }
+    }
+  }
+  B.s[C_FX][ci] = F[0];
+  B.s[C_FY][ci] = F[1];
+  B.s[C_FZ][ci] = d3 ? F[2] : 0.0;
+  B.s[C_TX][ci] = d3 ? acc[3] : 0.0;
+  B.s[C_TY][ci] = d3 ? acc[4] : 0.0;
+  B.s[C_TZ][ci] = acc[5];
+}
+
+// ---------------------------------------------------------------------------
+// integrators
+// ---------------------------------------------------------------------------
+template <class R, int DIM, bool WALL>
+__global__ __launch_bounds__(kBlock) void k_vv1(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int t = S.type[i];
+  if (t > 2) return;
+  double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
+  R v[3] = {S.v[0][i], S.v[1][i], DIM == 3 ? S.v[2][i] : R(0)};
+  if (t == 0) {
+    const R hdt = (R)(0.5 * P.dt);
+    const R hdtm = hdt / S.mass[i];
+    v[0] = v[0] + hdtm * D.f[0][i];
+    v[1] = v[1] + hdtm * D.f[1][i];
+    if (DIM == 3) v[2] = v[2] + hdtm * D.f[2][i];
+#pragma unroll
+    for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) S.c[k][i] = S.c[k][i] + hdt * D.dcdt[k][i];
+  } else {
+    v[0] = (R)(t == 1 ? P.V_bottom : P.V_top);
+    v[1] = R(0);
+    v[2] = R(0);
+  }
+  S.v[0][i] = v[0];
+  S.v[1][i] = v[1];
+  if (DIM == 3) S.v[2][i] = v[2];
+  xi = xi + (double)v[0] * P.dt;
+  yi = yi + (double)v[1] * P.dt;
+  if (DIM == 3) zi = zi + (double)v[2] * P.dt;
+  if (xi < 0) xi = xi + P.L[0];
+  if (!WALL && yi < 0) yi = yi + P.L[1];
+  if (DIM == 3 && zi < 0) zi = zi + P.L[2];
+  if (xi > P.L[0]) xi = xi - P.L[0];
+  if (!WALL && yi > P.L[1]) yi = yi - P.L[1];
+  if (DIM == 3 && zi > P.L[2]) zi = zi - P.L[2];
+  S.x[i] = xi;
+  S.y[i] = yi;
+  if (DIM == 3) S.z[i] = zi;
+}
+
+template <class R, int DIM>
+__global__ __launch_bounds__(kBlock) void k_vv2(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || S.type[i] != 0) return;
+  const R hdt = (R)(0.5 * P.dt);
+  const R hdtm = hdt / S.mass[i];
+  S.v[0][i] = S.v[0][i] + hdtm * D.f[0][i];
+  S.v[1][i] = S.v[1][i] + hdtm * D.f[1][i];
+  if (DIM == 3) S.v[2][i] = S.v[2][i] + hdtm * D.f[2][i];
+#pragma unroll
+  for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) S.c[k][i] = S.c[k][i] + hdt * D.dcdt[k][i];
+}
+
+__device__ __forceinline__ void quat_mul(const double c[4], const double d[4], double r[4]) {
+  // kernel_quaternion_product (src/kernel_quaternion_product.cu:15-32)
+  r[0] = c[0] * d[0] - c[1] * d[1] - c[2] * d[2] - c[3] * d[3];
+  r[1] = c[0] * d[1] + c[1] * d[0] + c[2] * d[3] - c[3] * d[2];
+  r[2] = c[0] * d[2] - c[1] * d[3] + c[2] * d[0] + c[3] * d[1];
+  r[3] = c[0] * d[3] + c[1] * d[2] - c[2] * d[1] + c[3] * d[0];
+}
+
+// Colloid half-kick, drift and rotation (one thread per colloid).  A 3D colloid
+// with |omega| == 0 keeps its orientation (the reference divides by zero).
+__global__ void k_coll_vv1(sph_params_t P, Bodies B) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B.n_colloids) return;
+  double **C = B.s;
+  const bool d3 = P.dim == 3;
+  const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
+  C[C_VX][i] += hdtm * C[C_FX][i];
+  C[C_VY][i] += hdtm * C[C_FY][i];
+  if (d3) C[C_VZ][i] += hdtm * C[C_FZ][i];
+  C[C_OZ][i] += hdtI * C[C_TZ][i];
+  double omega;
+  if (!d3) {
+    omega = C[C_OZ][i];
+  } else {
+    C[C_OX][i] += hdtI * C[C_TX][i];
+    C[C_OY][i] += hdtI * C[C_TY][i];
+    omega = sqrt(C[C_OX][i] * C[C_OX][i] + C[C_OY][i] * C[C_OY][i] + C[C_OZ][i] * C[C_OZ][i]);
+  }
+  double cx = C[C_X][i] + C[C_VX][i] * P.dt;
+  double cy = C[C_Y][i] + C[C_VY][i] * P.dt;
+  double cz = d3 ? C[C_Z][i] + C[C_VZ][i] * P.dt : 0.0;
+  if (cx < 0) cx = cx + P.L[0];
+  if (cx > P.L[0]) cx = cx - P.L[0];
+  if (P.wall == 0) {
+    if (cy < 0) cy = cy + P.L[1];
+    if (cy > P.L[1]) cy = cy - P.L[1];
+  }
+  if (d3) {
+    if (cz < 0) cz = cz + P.L[2];
+    if (cz > P.L[2]) cz = cz - P.L[2];
+  }
+  C[C_X][i] = cx;
+  C[C_Y][i] = cy;
+  if (d3) C[C_Z][i] = cz;
+  if (!d3) {
+    C[C_THETA][i] = C[C_THETA][i] + C[C_OZ][i] * P.dt;
+  } else {
+    double h = 0.5 * omega * P.dt;
+    double dq[4] = {1.0, 0.0, 0.0, 0.0};
+    if (omega != 0.0) {
+      double sh = sin(h);
+      dq[0] = cos(h);
+      dq[1] = sh * (C[C_OX][i] / omega);
+      dq[2] = sh * (C[C_OY][i] / omega);
+      dq[3] = sh * (C[C_OZ][i] / omega);
+    }
+    double q[4] = {C[C_Q0][i], C[C_QX][i], C[C_QY][i], C[C_QZ][i]}, qn[4];
+    quat_mul(dq, q, qn);
+    C[C_Q0][i] = qn[0];
+    C[C_QX][i] = qn[1];
+    C[C_QY][i] = qn[2];
+    C[C_QZ][i] = qn[3];
+  }
+}
+
+// Rigidly re-places every colloid boundary particle (one thread per list entry).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_coll_place(sph_params_t P, Particles<R> S, Bodies B,
+                                                       const int *__restrict__ rank, int nlist) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlist) return;
+  const int ci = B.cowner[j];
+  double *const *C = B.s;
+  double px, py, pz = 0.0;
+  if (P.dim == 2) {
+    double th = C[C_THETA][ci];
+    double sa = sin(th), ca = cos(th);
+    px = B.xc[j] * ca - B.yc[j] * sa;
+    py = B.xc[j] * sa + B.yc[j] * ca;
+  } else {
+    double q[4] = {C[C_Q0][ci], C[C_QX][ci], C[C_QY][ci], C[C_QZ][ci]};
+    double qi[4] = {q[0], -q[1], -q[2], -q[3]};
+    double pt[4] = {0.0, B.xc[j], B.yc[j], B.zc[j]}, t1[4], t2[4];
+    quat_mul(q, pt, t1);
+    quat_mul(t1, qi, t2);
+    px = t2[1];
+    py = t2[2];
+    pz = t2[3];
+  }
+  double X = px + C[C_X][ci], Y = py + C[C_Y][ci], Z = P.dim == 3 ? pz + C[C_Z][ci] : 0.0;
+  if (X < 0) X = X + P.L[0];
+  if (X > P.L[0]) X = X - P.L[0];
+  if (P.wall == 0) {
+    if (Y < 0) Y = Y + P.L[1];
+    if (Y > P.L[1]) Y = Y - P.L[1];
+  }
+  if (P.dim == 3) {
+    if (Z < 0) Z = Z + P.L[2];
+    if (Z > P.L[2]) Z = Z - P.L[2];
+  }
+  const int p = rank[B.clist[j]];
+  S.x[p] = X;
+  S.y[p] = Y;
+  if (P.dim == 3) S.z[p] = Z;
+}
+
+__global__ void k_coll_vv2(sph_params_t P, Bodies B) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B.n_colloids) return;
+  double **C = B.s;
+  const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
+  C[C_VX][i] += hdtm * C[C_FX][i];
+  C[C_VY][i] += hdtm * C[C_FY][i];
+  if (P.dim == 3) C[C_VZ][i] += hdtm * C[C_FZ][i];
+  C[C_OZ][i] += hdtI * C[C_TZ][i];
+  if (P.dim == 3) {
+    C[C_OX][i] += hdtI * C[C_TX][i];
+    C[C_OY][i] += hdtI * C[C_TY][i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kinetic energy: fixed partition + fixed reduction tree -> bitwise repeatable
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_ke_partial(sph_params_t P, Particles<R> S,
+                                                       double *part, int N) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+    double vx = S.v[0][i], vy = S.v[1][i], vz = P.dim == 3 ? (double)S.v[2][i] : 0.0;
+    acc[0] += 0.5 * (double)S.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  block_sum<1, kBlock>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_ke_final(const double *part, int n, double *out) {
+  double acc[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += kBlock) acc[0] += part[i];
+  block_sum<1, kBlock>(acc);
+  if (threadIdx.x == 0) out[0] = acc[0];
+}
+
+// ---------------------------------------------------------------------------
+// owner of the device state
+// ---------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    n = count;
+    return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+};
+
+template <class R>
+class DeviceImpl final : public Device {
+ public:
+  ~DeviceImpl() override {
+    for (auto &e : events_) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int upload(const HostState &hs, std::string &err) override {
+    P_ = hs.p;
+    N_ = P_.N;
+    nc_ = P_.dim == 3 ? 6 : 3;
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(x_[b].alloc(N_)); HIPCHK(y_[b].alloc(N_)); HIPCHK(z_[b].alloc(N_));
+      for (int a = 0; a < 3; ++a) HIPCHK(v_[b][a].alloc(N_));
+      for (int a = 0; a < 6; ++a) HIPCHK(c_[b][a].alloc(N_));
+      HIPCHK(mass_[b].alloc(N_)); HIPCHK(type_[b].alloc(N_)); HIPCHK(id_[b].alloc(N_));
+    }
+    HIPCHK(dens_.alloc(N_)); HIPCHK(press_.alloc(N_));
+    for (int a = 0; a < 3; ++a) HIPCHK(f_[a].alloc(N_));
+    for (int a = 0; a < 6; ++a) HIPCHK(dcdt_[a].alloc(N_));
+    for (int a = 0; a < 9; ++a) HIPCHK(gv_[a].alloc(N_));
+    HIPCHK(cell_.alloc(N_)); HIPCHK(slot_.alloc(N_)); HIPCHK(rank_.alloc(N_));
+    HIPCHK(stage_src_.alloc(N_)); HIPCHK(stage_id_.alloc(N_)); HIPCHK(sorted_cell_.alloc(N_));
+    HIPCHK(count_.alloc(P_.Ntotal_cells)); HIPCHK(cs_.alloc(P_.Ntotal_cells + 1));
+    HIPCHK(flag_.alloc(1));
+    HIPCHK(ke_part_.alloc(kKeBlocks)); HIPCHK(ke_out_.alloc(1));
+    const int Nc = P_.N_colloids;
+    for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(Nc));
+    HIPCHK(clist_.alloc(P_.Nlist_colloids)); HIPCHK(cstart_.alloc(Nc + 1));
+    HIPCHK(cowner_.alloc(P_.Nlist_colloids));
+    HIPCHK(xc_.alloc(P_.Nlist_colloids)); HIPCHK(yc_.alloc(P_.Nlist_colloids));
+    HIPCHK(zc_.alloc(P_.Nlist_colloids));
+    HIPCHK(wlist_.alloc(P_.Nlist_walls)); HIPCHK(wstart_.alloc(std::max(P_.Nwalls, 1)));
+    HIPCHK(fwall_.alloc(6));
+
+    // particle state, ID order = initial order
+    for (int k : {F_X, F_Y, F_Z}) {
+      DevBuf<double> &dst = k == F_X ? x_[0] : (k == F_Y ? y_[0] : z_[0]);
+      HIPCHK(hipMemcpy(dst.p, hs.pf[k].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int rc = put_real(v_[0][0].p, hs.pf[F_VX], nullptr, err); if (rc) return rc;
+    rc = put_real(v_[0][1].p, hs.pf[F_VY], nullptr, err); if (rc) return rc;
+    rc = put_real(v_[0][2].p, hs.pf[F_VZ], nullptr, err); if (rc) return rc;
+    static const int cfield[6] = {F_CXX, F_CXY, F_CYY, F_CXZ, F_CYZ, F_CZZ};
+    for (int a = 0; a < 6; ++a) {
+      rc = put_real(c_[0][a].p, hs.pf[cfield[a]], nullptr, err); if (rc) return rc;
+    }
+    rc = put_real(mass_[0].p, hs.pf[F_MASS], nullptr, err); if (rc) return rc;
+    HIPCHK(hipMemcpy(type_[0].p, hs.type.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> iota(N_);
+    std::iota(iota.begin(), iota.end(), 0);
+    HIPCHK(hipMemcpy(id_[0].p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(rank_.p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
+    id_host_ = iota;
+    // derived fields start at zero (reference: cudaMemset, system_copy_pointers_to_device.cpp:190-256)
+    static const int ffield[3] = {F_FX, F_FY, F_FZ};
+    for (int a = 0; a < 3; ++a) {
+      rc = put_real(f_[a].p, hs.pf[ffield[a]], nullptr, err); if (rc) return rc;
+    }
+    for (int a = 0; a < 6; ++a) HIPCHK(hipMemset(dcdt_[a].p, 0, N_ * sizeof(R)));
+    for (int a = 0; a < 9; ++a) HIPCHK(hipMemset(gv_[a].p, 0, N_ * sizeof(R)));
+    HIPCHK(hipMemset(dens_.p, 0, N_ * sizeof(R)));
+    HIPCHK(hipMemset(press_.p, 0, N_ * sizeof(R)));
+    HIPCHK(hipMemset(count_.p, 0, P_.Ntotal_cells * sizeof(int)));
+    HIPCHK(hipMemset(cs_.p, 0, (P_.Ntotal_cells + 1) * sizeof(int)));
+    HIPCHK(hipMemset(flag_.p, 0, sizeof(int)));
+    HIPCHK(hipMemset(fwall_.p, 0, 6 * sizeof(double)));
+    for (int k = 0; k < C_NUM; ++k)
+      if (Nc) HIPCHK(hipMemcpy(coll_[k].p, hs.cf[k].data(), Nc * sizeof(double),
+                               hipMemcpyHostToDevice));
+    if (P_.Nlist_colloids) {
+      HIPCHK(hipMemcpy(clist_.p, hs.colloids_list.data(), P_.Nlist_colloids * sizeof(int),
+                       hipMemcpyHostToDevice));
+      std::vector<int> owner(P_.Nlist_colloids);
+      std::vector<int> cst(Nc + 1);
+      for (int i = 0; i < Nc; ++i) cst[i] = hs.colloids_start[i];
+      cst[Nc] = P_.Nlist_colloids;
+      for (int i = 0; i < Nc; ++i)
+        for (int j = cst[i]; j < cst[i + 1]; ++j) owner[j] = i;
+      HIPCHK(hipMemcpy(cstart_.p, cst.data(), (Nc + 1) * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(cowner_.p, owner.data(), P_.Nlist_colloids * sizeof(int),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(xc_.p, hs.x_center.data(), P_.Nlist_colloids * sizeof(double),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(yc_.p, hs.y_center.data(), P_.Nlist_colloids * sizeof(double),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(zc_.p, hs.z_center.data(), P_.Nlist_colloids * sizeof(double),
+                       hipMemcpyHostToDevice));
+    } else if (Nc) {
+      std::vector<int> cst(Nc + 1, 0);
+      HIPCHK(hipMemcpy(cstart_.p, cst.data(), (Nc + 1) * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (P_.Nlist_walls)
+      HIPCHK(hipMemcpy(wlist_.p, hs.walls_list.data(), P_.Nlist_walls * sizeof(int),
+                       hipMemcpyHostToDevice));
+    if (P_.Nwalls)
+      HIPCHK(hipMemcpy(wstart_.p, hs.walls_start.data(), P_.Nwalls * sizeof(int),
+                       hipMemcpyHostToDevice));
+    cur_ = 0;
+    have_cells_ = false;
+    HIPCHK(hipDeviceSynchronize());
+    return SPH_OK;
+  }
+
+  // ---- per-step phases ----
+  int neighbours_search(std::string &err) override {
+    Mark m(this, PH_NS);
+    const int nb = blocks(N_);
+    Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
+    k_hash<<<nb, kBlock, 0, stream_>>>(P_, S.x, S.y, S.z, cell_.p, slot_.p, count_.p, flag_.p, N_);
+    k_scan_cells<<<1, 1024, 0, stream_>>>(count_.p, cs_.p, P_.Ntotal_cells);
+    k_scatter<<<nb, kBlock, 0, stream_>>>(cell_.p, slot_.p, cs_.p, S.id, stage_src_.p,
+                                          stage_id_.p, N_);
+    k_rank_gather<R><<<nb, kBlock, 0, stream_>>>(S, T, cell_.p, cs_.p, stage_src_.p, stage_id_.p,
+                                                 rank_.p, sorted_cell_.p, P_.Ntotal_cells, nc_,
+                                                 N_);
+    HIPCHK(hipGetLastError());
+    cur_ ^= 1;
+    have_cells_ = true;
+    id_dirty_ = true;
+    return SPH_OK;
+  }
+
+  int density(std::string &err) override {
+    Mark m(this, PH_DENS);
+    dispatch3([&](auto dimc, auto wallc) {
+      constexpr int DIM = decltype(dimc)::value;
+      constexpr bool WALL = decltype(wallc)::value;
+      k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(),
+                                                                  cs_.p, N_);
+    });
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int pressure(std::string &err) override {
+    k_pressure<R><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int vel_gradient(bool store_gradv, std::string &err) override {
+    // store_gradv == true: the reference's calculate_vel_gradient output only.
+    return gradv_impl(store_gradv ? 1 : 0, store_gradv ? 0 : 1, err);
+  }
+  int gradv_dcdt(bool store_gradv, std::string &err) {
+    return gradv_impl(store_gradv ? 1 : 0, 1, err);
+  }
+  int gradv_impl(int store, int dcdt, std::string &err) {
+    Mark m(this, PH_GRADV);
+    dispatch3([&](auto dimc, auto wallc) {
+      constexpr int DIM = decltype(dimc)::value;
+      constexpr bool WALL = decltype(wallc)::value;
+      k_gradv<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(),
+                                                                bodies(), cs_.p, N_, store, dcdt);
+    });
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int pair_forces(bool with_ext, std::string &err) override {
+    Mark m(this, PH_FORCES);
+    dispatch3([&](auto dimc, auto wallc) {
+      constexpr int DIM = decltype(dimc)::value;
+      constexpr bool WALL = decltype(wallc)::value;
+      k_forces<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
+          P_, parts(cur_), derived(), bodies(), cs_.p, N_, with_ext ? 1 : 0);
+    });
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int external_forces(std::string &err) override {
+    if (P_.ext_force_type == 0) return SPH_OK;
+    if (P_.dim == 3)
+      k_ext<R, 3><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    else
+      k_ext<R, 2><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int force_walls(std::string &err) override {
+    if (!P_.wall || P_.Nwalls == 0) return SPH_OK;
+    Mark m(this, PH_BODIES);
+    k_walls<R><<<P_.Nwalls, kRedBlock, 0, stream_>>>(P_, derived(), bodies(), rank_.p);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int force_colloids(std::string &err) override {
+    if (P_.N_colloids == 0) return SPH_OK;
+    Mark m(this, PH_BODIES);
+    k_colloids<R><<<P_.N_colloids, kBlock, 0, stream_>>>(P_, parts(cur_), derived(), bodies(),
+                                                         rank_.p, mag_[0], mag_[1], mag_[2]);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int vv1(std::string &err) override {
+    Mark m(this, PH_VV1);
+    dispatch3([&](auto dimc, auto wallc) {
+      constexpr int DIM = decltype(dimc)::value;
+      constexpr bool WALL = decltype(wallc)::value;
+      k_vv1<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    });
+    if (P_.N_colloids > 0 && P_.coll_move == 0) {
+      k_coll_vv1<<<blocks(P_.N_colloids), kBlock, 0, stream_>>>(P_, bodies());
+      if (P_.Nlist_colloids > 0)
+        k_coll_place<R><<<blocks(P_.Nlist_colloids), kBlock, 0, stream_>>>(
+            P_, parts(cur_), bodies(), rank_.p, P_.Nlist_colloids);
+    }
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  int vv2(std::string &err) override {
+    Mark m(this, PH_VV2);
+    if (P_.dim == 3)
+      k_vv2<R, 3><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    else
+      k_vv2<R, 2><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    if (P_.coll_move == 0 && P_.N_colloids > 0)
+      k_coll_vv2<<<blocks(P_.N_colloids), kBlock, 0, stream_>>>(P_, bodies());
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
+  void set_magnetic_moment(const double m[3]) override {
+    mag_[0] = m[0];
+    mag_[1] = m[1];
+    mag_[2] = m[2];
+  }
+
+  int kinetic_energy(double *ke, std::string &err) override {
+    k_ke_partial<R><<<kKeBlocks, kBlock, 0, stream_>>>(P_, parts(cur_), ke_part_.p, N_);
+    k_ke_final<<<1, kBlock, 0, stream_>>>(ke_part_.p, kKeBlocks, ke_out_.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ke, ke_out_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (!std::isfinite(*ke)) {
+      err = "kinetic energy is not finite";
+      return SPH_ERR_NAN;
+    }
+    return SPH_OK;
+  }
+
+  int synchronize(std::string &err) override {
+    HIPCHK(hipStreamSynchronize(stream_));
+    return check_flags(err);
+  }
+
+  int check_flags(std::string &err) override {
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, flag_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (flag) {
+      HIPCHK(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
+      HIPCHK(hipStreamSynchronize(stream_));
+      err = "kernel cell ranges error: a particle has not a proper cell assigned";
+      return SPH_ERR_OUT_OF_GRID;
+    }
+    return SPH_OK;
+  }
+
+  // ---- field exchange (ID order on the host side) ----
+  int download_field(HostState &hs, const std::string &name, std::string &err) override {
+    HIPCHK(hipStreamSynchronize(stream_));
+    int rc = refresh_ids(err);
+    if (rc) return rc;
+    for (int k = 0; k < F_NUM; ++k) {
+      if (name != kPFieldNames[k]) continue;
+      std::vector<double> &dst = hs.pf[k];
+      dst.assign(std::max(hs.p.Nmax, N_), 0.0);
+      if (k == F_X || k == F_Y || k == F_Z) {
+        std::vector<double> tmp(N_);
+        const double *src = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
+        HIPCHK(hipMemcpy(tmp.data(), src, N_ * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < N_; ++p) dst[id_host_[p]] = tmp[p];
+        return SPH_OK;
+      }
+      const R *src = real_ptr(k);
+      std::vector<R> tmp(N_);
+      HIPCHK(hipMemcpy(tmp.data(), src, N_ * sizeof(R), hipMemcpyDeviceToHost));
+      for (int p = 0; p < N_; ++p) dst[id_host_[p]] = (double)tmp[p];
+      return SPH_OK;
+    }
+    if (name == "type") {
+      std::vector<int> tmp(N_);
+      HIPCHK(hipMemcpy(tmp.data(), type_[cur_].p, N_ * sizeof(int), hipMemcpyDeviceToHost));
+      hs.type.assign(std::max(hs.p.Nmax, N_), 0);
+      for (int p = 0; p < N_; ++p) hs.type[id_host_[p]] = tmp[p];
+      return SPH_OK;
+    }
+    for (int k = 0; k < C_NUM; ++k) {
+      if (name != kCFieldNames[k]) continue;
+      hs.cf[k].assign(P_.N_colloids, 0.0);
+      if (P_.N_colloids)
+        HIPCHK(hipMemcpy(hs.cf[k].data(), coll_[k].p, P_.N_colloids * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      return SPH_OK;
+    }
+    if (name == "fx_wall" || name == "fy_wall" || name == "fz_wall") {
+      double fw[6];
+      HIPCHK(hipMemcpy(fw, fwall_.p, 6 * sizeof(double), hipMemcpyDeviceToHost));
+      int a = name[1] - 'x';
+      hs.fwall[a][0] = fw[a * 2 + 0];
+      hs.fwall[a][1] = fw[a * 2 + 1];
+      return SPH_OK;
+    }
+    return SPH_OK;  // lists and body offsets never change on the device
+  }
+
+  int upload_field(const HostState &hs, const std::string &name, std::string &err) override {
+    HIPCHK(hipStreamSynchronize(stream_));
+    int rc = refresh_ids(err);
+    if (rc) return rc;
+    for (int k = 0; k < F_NUM; ++k) {
+      if (name != kPFieldNames[k]) continue;
+      if (k == F_X || k == F_Y || k == F_Z) {
+        std::vector<double> tmp(N_);
+        for (int p = 0; p < N_; ++p) tmp[p] = hs.pf[k][id_host_[p]];
+        double *dst = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
+        HIPCHK(hipMemcpy(dst, tmp.data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+        return SPH_OK;
+      }
+      return put_real(real_ptr(k), hs.pf[k], &id_host_, err);
+    }
+    if (name == "type") {
+      std::vector<int> tmp(N_);
+      for (int p = 0; p < N_; ++p) tmp[p] = hs.type[id_host_[p]];
+      HIPCHK(hipMemcpy(type_[cur_].p, tmp.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
+      return SPH_OK;
+    }
+    for (int k = 0; k < C_NUM; ++k) {
+      if (name != kCFieldNames[k]) continue;
+      if (P_.N_colloids)
+        HIPCHK(hipMemcpy(coll_[k].p, hs.cf[k].data(), P_.N_colloids * sizeof(double),
+                         hipMemcpyHostToDevice));
+      return SPH_OK;
+    }
+    err = "field '" + name + "' cannot be written on the device";
+    return SPH_ERR_ARG;
+  }
+
+  int cell_list(int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce, std::string &err) override {
+    if (!have_cells_) {
+      err = "no neighbours_search has run";
+      return SPH_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (pc) HIPCHK(hipMemcpy(pc, sorted_cell_.p, N_ * sizeof(int), hipMemcpyDeviceToHost));
+    if (pi) HIPCHK(hipMemcpy(pi, id_[cur_].p, N_ * sizeof(int), hipMemcpyDeviceToHost));
+    if (cs || ce) {
+      std::vector<int> tmp(P_.Ntotal_cells + 1);
+      HIPCHK(hipMemcpy(tmp.data(), cs_.p, tmp.size() * sizeof(int), hipMemcpyDeviceToHost));
+      for (int c = 0; c < P_.Ntotal_cells; ++c) {
+        if (cs) cs[c] = tmp[c];
+        if (ce) ce[c] = tmp[c + 1] - 1;
+      }
+    }
+    return SPH_OK;
+  }
+
+  void set_timing(bool on) override { timing_ = on; }
+
+  int get_timing(double ms[8], int64_t calls[8], std::string &err) override {
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int k = 0; k < 8; ++k) { ms[k] = 0.0; calls[k] = 0; }
+    for (auto &e : events_) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e.a, e.b));
+      ms[e.phase] += t;
+      calls[e.phase] += 1;
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    events_.clear();
+    return SPH_OK;
+  }
+
+ private:
+  struct Ev { hipEvent_t a, b; int phase; };
+  struct Mark {  // brackets one phase with events when timing is on
+    DeviceImpl *d;
+    int phase;
+    hipEvent_t a = nullptr, b = nullptr;
+    Mark(DeviceImpl *dd, int ph) : d(dd), phase(ph) {
+      if (!d->timing_) return;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+      (void)hipEventRecord(a, d->stream_);
+    }
+    ~Mark() {
+      if (!a) return;
+      (void)hipEventRecord(b, d->stream_);
+      d->events_.push_back({a, b, phase});
+    }
+  };
+
+  static int blocks(int n) { return std::max(1, (n + kBlock - 1) / kBlock); }
+
+  template <class Fn>
+  void dispatch3(Fn &&fn) {
+    std::integral_constant<int, 2> d2;
+    std::integral_constant<int, 3> d3;
+    std::true_type w1;
+    std::false_type w0;
+    if (P_.dim == 3) {
+      if (P_.wall) fn(d3, w1); else fn(d3, w0);
+    } else {
+      if (P_.wall) fn(d2, w1); else fn(d2, w0);
+    }
+  }
+
+  Particles<R> parts(int b) {
+    Particles<R> S;
+    S.x = x_[b].p; S.y = y_[b].p; S.z = z_[b].p;
+    for (int a = 0; a < 3; ++a) S.v[a] = v_[b][a].p;
+    for (int a = 0; a < 6; ++a) S.c[a] = c_[b][a].p;
+    S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
+    return S;
+  }
+  Derived<R> derived() {
+    Derived<R> D;
+    D.dens = dens_.p; D.press = press_.p;
+    for (int a = 0; a < 3; ++a) D.f[a] = f_[a].p;
+    for (int a = 0; a < 6; ++a) D.dcdt[a] = dcdt_[a].p;
+    for (int a = 0; a < 9; ++a) D.gv[a] = gv_[a].p;
+    return D;
+  }
+  Bodies bodies() {
+    Bodies B;
+    for (int k = 0; k < C_NUM; ++k) B.s[k] = coll_[k].p;
+    B.clist = clist_.p; B.cstart = cstart_.p; B.cowner = cowner_.p;
+    B.xc = xc_.p; B.yc = yc_.p; B.zc = zc_.p;
+    B.wlist = wlist_.p; B.wstart = wstart_.p; B.fwall = fwall_.p;
+    B.n_colloids = P_.N_colloids;
+    return B;
+  }
+
+  R *real_ptr(int k) {
+    switch (k) {
+      case F_VX: return v_[cur_][0].p;
+      case F_VY: return v_[cur_][1].p;
+      case F_VZ: return v_[cur_][2].p;
+      case F_MASS: return mass_[cur_].p;
+      case F_DENS: return dens_.p;
+      case F_PRESS: return press_.p;
+      case F_FX: return f_[0].p;
+      case F_FY: return f_[1].p;
+      case F_FZ: return f_[2].p;
+      case F_CXX: return c_[cur_][0].p;
+      case F_CXY: return c_[cur_][1].p;
+      case F_CYY: return c_[cur_][2].p;
+      case F_CXZ: return c_[cur_][3].p;
+      case F_CYZ: return c_[cur_][4].p;
+      case F_CZZ: return c_[cur_][5].p;
+      default:
+        if (k >= F_DCDT_XX && k <= F_DCDT_ZZ) return dcdt_[k - F_DCDT_XX].p;
+        if (k >= F_GXX && k <= F_GZZ) return gv_[k - F_GXX].p;
+        return nullptr;
+    }
+  }
+
+  // Writes host values (ID order) into a device array in the current order.
+  int put_real(R *dst, const std::vector<double> &src, const std::vector<int> *ids,
+               std::string &err) {
+    std::vector<R> tmp(N_);
+    for (int p = 0; p < N_; ++p) tmp[p] = (R)src[ids ? (*ids)[p] : p];
+    HIPCHK(hipMemcpy(dst, tmp.data(), N_ * sizeof(R), hipMemcpyHostToDevice));
+    return SPH_OK;
+  }
+
+  int refresh_ids(std::string &err) {
+    if (!id_dirty_) return SPH_OK;
+    id_host_.resize(N_);
+    HIPCHK(hipMemcpy(id_host_.data(), id_[cur_].p, N_ * sizeof(int), hipMemcpyDeviceToHost));
+    id_dirty_ = false;
+    return SPH_OK;
+  }
+
+  sph_params_t P_{};
+  int N_ = 0, nc_ = 3, cur_ = 0;
+  bool have_cells_ = false, id_dirty_ = false, timing_ = false;
+  double mag_[3] = {0, 0, 0};
+  hipStream_t stream_ = nullptr;
+  std::vector<Ev> events_;
+  std::vector<int> id_host_;
+
+  DevBuf<double> x_[2], y_[2], z_[2];
+  DevBuf<R> v_[2][3], c_[2][6], mass_[2];
+  DevBuf<int> type_[2], id_[2];
+  DevBuf<R> dens_, press_, f_[3], dcdt_[6], gv_[9];
+  DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, flag_;
+  DevBuf<double> ke_part_, ke_out_;
+  DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_;
+  DevBuf<int> clist_, cstart_, cowner_, wlist_, wstart_;
+
+ public:
+  // calculate_forces as one sequence (src/system_calculate_forces.cu:86-170).
+  int forces_all(bool store_gradv, std::string &err) {
+    int rc;
+    if ((rc = density(err))) return rc;
+    if ((rc = gradv_dcdt(store_gradv, err))) return rc;
+    if ((rc = pair_forces(true, err))) return rc;
+    if ((rc = force_walls(err))) return rc;
+    return force_colloids(err);
+  }
+};
+
+}  // namespace
+
+// Exposed for the C-ABI layer: the fused calculate_forces sequence.
+int device_forces_all(Device *d, int precision, bool store_gradv, std::string &err) {
+  if (precision == SPH_FP32)
+    return static_cast<DeviceImpl<float> *>(d)->forces_all(store_gradv, err);
+  return static_cast<DeviceImpl<double> *>(d)->forces_all(store_gradv, err);
+}
+
+std::unique_ptr<Device> make_device(int precision) {
+  if (precision == SPH_FP32) return std::unique_ptr<Device>(new DeviceImpl<float>());
+  return std::unique_ptr<Device>(new DeviceImpl<double>());
+}
+
+}  // namespace sph

@@ -1,0 +1,177 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 oracle.
+
+Tolerances (normwise relative error ||gpu - oracle|| / ||oracle|| unless noted):
+  cell list, sorted order, cell ranges           bit-exact (both precision forms)
+  FP64 form, after one step: dens, gradv, press, F, dc/dt, wall and colloid
+      force/torque                                1e-12
+  FP64 form, positions after steps                1e-12 absolute (box length 1)
+  FP32 form (pair arithmetic in float): dens 1e-6, gradv and dc/dt 1e-4,
+      F 1e-3, wall/colloid sums 1e-3, positions 1e-9 absolute after one step
+"""
+import numpy as np
+import pytest
+
+import helpers
+from helpers import comps, relerr
+import oracle
+import sph_system
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "couette2d": lambda: helpers.cfg_couette(n=64),
+    "sine3d": lambda: helpers.cfg_sine(n=24),
+    "colloids3d": lambda: helpers.cfg_colloids_3d(n=30),
+    "colloids2d": lambda: helpers.cfg_colloids_2d(n=48),
+}
+
+
+def _pair(case, tmpdir, precision=sph_system.FP64):
+    s = helpers.make_system(CASES[case](), tmpdir, precision)
+    o = oracle.Oracle.from_system(s)
+    return s, o
+
+
+def _check_fields(s, o, tol, names):
+    for n in names:
+        e = relerr(s.get(n), o.get(n))
+        assert e <= tol, f"{n}: rel err {e:.3e} > {tol:.1e}"
+
+
+def _check_cells(s, o):
+    pc, pi, cs, ce = s.cell_list()
+    opc, opi, ocs, oce = o.cell_list()
+    np.testing.assert_array_equal(pc, opc)
+    np.testing.assert_array_equal(pi, opi)
+    occupied = oce >= ocs
+    np.testing.assert_array_equal(cs[occupied], ocs[occupied])
+    np.testing.assert_array_equal(ce[occupied], oce[occupied])
+    assert np.all(ce[~occupied] < cs[~occupied])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision", [sph_system.FP64, sph_system.FP32])
+def test_cell_list_bit_exact(case, precision, tmp_path):
+    s, o = _pair(case, str(tmp_path), precision)
+    p = s.params
+    rng = np.random.default_rng(7)
+    # Shuffle particles across cells, and put some exactly on cell faces and box faces.
+    L = np.array(p.L)
+    for a, n in enumerate(["x", "y", "z"][:p.dim]):
+        v = s.get(n)
+        t = s.get("type")
+        move = t == 0
+        v[move] = v[move] + rng.uniform(-0.3, 0.3, move.sum()) * p.rcut
+        if not (a == 1 and p.wall):
+            v = np.mod(v, L[a])
+            k = rng.choice(np.flatnonzero(move), 8, replace=False)
+            v[k[:3]] = 0.0
+            v[k[3:5]] = L[a]
+            v[k[5:]] = p.cell_size[a] * rng.integers(1, p.Ncells[a], 3)
+        s.set(n, v)
+        o.set(n, v)
+    s.neighbours_search()
+    o.neighbours_search()
+    _check_cells(s, o)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_prepare_fp64(case, tmp_path):
+    s, o = _pair(case, str(tmp_path))
+    s.prepare()
+    o.prepare()
+    _check_cells(s, o)
+    d = s.params.dim
+    names = (["dens", "press"] + comps("gradv", d) + comps("f", d) + comps("dcdt", d))
+    for n in names:
+        e = relerr(s.get(n), o.get(n))
+        assert e <= 1e-12, f"{case} {n}: {e:.3e}"
+    if s.params.wall:
+        _check_fields(s, o, 1e-12, ["fx_wall", "fy_wall"])
+    if s.params.N_colloids:
+        _check_fields(s, o, 1e-12, ["fx_colloid", "fy_colloid", "tz_colloid"] +
+                      (["fz_colloid", "tx_colloid", "ty_colloid"] if d == 3 else []))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_steps_fp64(case, tmp_path):
+    s, o = _pair(case, str(tmp_path))
+    s.prepare()
+    o.prepare()
+    nsteps = 5
+    s.step(nsteps)
+    o.step(nsteps)
+    s.synchronize()
+    p = s.params
+    d = p.dim
+    for a, n in enumerate(comps("x", d)):
+        diff = helpers.periodic_diff(s.get(n), o.get(n), p.L[a])
+        assert np.max(np.abs(diff)) <= 1e-12, n
+    for grp in ("v", "c", "f", "dcdt"):
+        for n in comps(grp, d):
+            e = relerr(s.get(n), o.get(n))
+            assert e <= 1e-10, f"{case} {n}: {e:.3e}"
+    _check_fields(s, o, 1e-10, ["dens", "press"])
+    ke, oke = s.kinetic_energy(), o.kinetic_energy()
+    assert abs(ke - oke) <= 1e-12 * abs(oke)
+    if p.N_colloids:
+        _check_fields(s, o, 1e-9, ["coll_x", "coll_y", "coll_vx", "coll_vy", "coll_omegaz",
+                                   "fx_colloid", "fy_colloid", "tz_colloid"])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp32_form(case, tmp_path):
+    s, o = _pair(case, str(tmp_path), sph_system.FP32)
+    s.prepare()
+    o.prepare()
+    _check_cells(s, o)
+    d = s.params.dim
+    _check_fields(s, o, 1e-6, ["dens"])
+    _check_fields(s, o, 1e-4, comps("gradv", d) + comps("dcdt", d))
+    _check_fields(s, o, 1e-3, comps("f", d))
+    if s.params.wall:
+        _check_fields(s, o, 1e-3, ["fx_wall", "fy_wall"])
+    if s.params.N_colloids:
+        _check_fields(s, o, 1e-3, ["fx_colloid", "fy_colloid", "tz_colloid"])
+    s.step(1)
+    o.step(1)
+    s.synchronize()
+    p = s.params
+    for a, n in enumerate(comps("x", d)):
+        diff = helpers.periodic_diff(s.get(n), o.get(n), p.L[a])
+        assert np.max(np.abs(diff)) <= 1e-9, n
+    _check_fields(s, o, 1e-3, comps("f", d))
+
+
+@pytest.mark.parametrize("precision", [sph_system.FP64, sph_system.FP32])
+def test_bitwise_repeatable(precision, tmp_path):
+    outs = []
+    for rep in range(2):
+        s = helpers.make_system(helpers.cfg_colloids_3d(n=24), str(tmp_path), precision,
+                                name=f"in{rep}")
+        s.prepare()
+        s.step(3)
+        s.synchronize()
+        outs.append({n: s.get(n) for n in ["x", "vx", "fx", "dens", "fx_colloid", "fx_wall"]}
+                    | {"ke": np.array([s.kinetic_energy()])})
+    for n in outs[0]:
+        np.testing.assert_array_equal(outs[0][n], outs[1][n], err_msg=n)
+
+
+def test_method_sequence_matches_step(tmp_path):
+    """VV1, neighbours_search, set_step, calculate_forces, VV2 called one by one
+    give the same state as the fused sph_step."""
+    a = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="a")
+    b = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="b")
+    a.prepare()
+    b.prepare()
+    for step in (1, 2):
+        a.V_Verlet_step1()
+        a.neighbours_search()
+        a.set_step(step)
+        a.calculate_forces()
+        a.V_Verlet_step2()
+    b.step(2)
+    b.synchronize()
+    for n in ["x", "y", "vx", "vy", "cxy", "fx", "coll_x", "tz_colloid"]:
+        np.testing.assert_array_equal(a.get(n), b.get(n), err_msg=n)
