@@ -130,6 +130,9 @@ template <int DIM, bool WALL, class Fn>
 __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int *__restrict__ cs,
                                                int cx, int cy, int cz, Fn &&fn) {
   const int Nx = P.Ncells[0], Ny = P.Ncells[1], Nz = P.Ncells[2];
+  // A particle outside the grid (already flagged by k_hash) has no neighbours;
+  // this keeps every cs[] index in bounds.
+  if (cx < 0 || cx > Nx || cy < -1 || cy > Ny || (DIM == 3 && (cz < 0 || cz > Nz))) return;
 #pragma unroll
   for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
     int z = cz + dz;
@@ -1055,6 +1058,18 @@ class DeviceImpl final : public Device {
     }
     for (int a = 0; a < 6; ++a) HIPCHK(hipMemset(dcdt_[a].p, 0, N_ * sizeof(R)));
     for (int a = 0; a < 9; ++a) HIPCHK(hipMemset(gv_[a].p, 0, N_ * sizeof(R)));
+    // The second buffer set starts as a copy, so rows a failed reorder leaves
+    // unwritten still hold finite values.
+    HIPCHK(hipMemcpy(x_[1].p, x_[0].p, N_ * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(y_[1].p, y_[0].p, N_ * sizeof(double), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(z_[1].p, z_[0].p, N_ * sizeof(double), hipMemcpyDeviceToDevice));
+    for (int a = 0; a < 3; ++a)
+      HIPCHK(hipMemcpy(v_[1][a].p, v_[0][a].p, N_ * sizeof(R), hipMemcpyDeviceToDevice));
+    for (int a = 0; a < 6; ++a)
+      HIPCHK(hipMemcpy(c_[1][a].p, c_[0][a].p, N_ * sizeof(R), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(mass_[1].p, mass_[0].p, N_ * sizeof(R), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(type_[1].p, type_[0].p, N_ * sizeof(int), hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(id_[1].p, id_[0].p, N_ * sizeof(int), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemset(dens_.p, 0, N_ * sizeof(R)));
     HIPCHK(hipMemset(press_.p, 0, N_ * sizeof(R)));
     HIPCHK(hipMemset(count_.p, 0, P_.Ntotal_cells * sizeof(int)));

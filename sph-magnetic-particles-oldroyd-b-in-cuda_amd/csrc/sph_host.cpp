@@ -14,7 +14,10 @@
 //   * colloid boundary particles get mass = 0, v = 0, c = 0 (reference leaves them
 //     uninitialised, new_sim:115 vs 155-244);
 //   * every appended colloid-shell particle is checked against Nmax (reference checks
-//     only the 3D north pole, new_sim:190).
+//     only the 3D north pole, new_sim:190);
+//   * the 3D shell poles are wrapped into the box like the ring particles (reference
+//     leaves them unwrapped, new_sim:194-196, 241-243, so a colloid within R of z = 0
+//     or z = L puts a pole outside the cell grid and trips its device assert).
 #include "sph_host.h"
 
 #include <cmath>
@@ -407,8 +410,13 @@ int particles_new_sim(HostState &s, const double dx[3], std::string &err) {
       double r = 0.5 * dx[0];
       for (int j = 0; j < Nlayers; ++j) {
         if (r < R && r > R - p.wall_width) {
-          // North pole (not wrapped, as in the reference).
-          if (!append(cx[i], cy[i], cz[i] + r, 3 + i)) return SPH_ERR_GEOMETRY;
+          auto wrap_z = [&](double zi) {
+            if (zi < 0) zi = zi + L2;
+            if (zi > L2) zi = zi - L2;
+            return zi;
+          };
+          // North pole.
+          if (!append(cx[i], cy[i], wrap_z(cz[i] + r), 3 + i)) return SPH_ERR_GEOMETRY;
           double d_theta = dx[0] / r;
           int Npart_theta = (int)std::round(M_PI / d_theta);
           d_theta = M_PI / Npart_theta;
@@ -430,8 +438,8 @@ int particles_new_sim(HostState &s, const double dx[3], std::string &err) {
             }
             theta = theta + d_theta;
           }
-          // South pole (not wrapped).
-          if (!append(cx[i], cy[i], cz[i] - r, 3 + i)) return SPH_ERR_GEOMETRY;
+          // South pole.
+          if (!append(cx[i], cy[i], wrap_z(cz[i] - r), 3 + i)) return SPH_ERR_GEOMETRY;
         }
         r = r + dx[0];
       }
@@ -753,18 +761,20 @@ int append_macro(const HostState &s, int step, double ke, const std::string &dir
 int write_restart(const HostState &s, const std::string &path, std::string &err) {
   FILE *f = std::fopen(path.c_str(), "w");
   if (!f) { err = "write restart: cannot open " + path; return SPH_ERR_IO; }
-  const auto &P = s.pf;
+  // The reader parses floats, so values are rounded to float first and printed
+  // with 9 significant digits, which round-trips every float exactly.
+  auto P = [&](int k, int i) { return (double)(float)s.pf[k][i]; };
   for (int i = 0; i < s.p.N; ++i) {
     if (s.p.dim == 2)
-      std::fprintf(f, "%.9g %.9g %.9g %.9g %.9g %d %.9g %.9g %.9g\n", P[F_X][i], P[F_Y][i],
-                   P[F_VX][i], P[F_VY][i], P[F_MASS][i], s.type[i], P[F_CXX][i],
-                   P[F_CXY][i], P[F_CYY][i]);
+      std::fprintf(f, "%.9g %.9g %.9g %.9g %.9g %d %.9g %.9g %.9g\n", P(F_X, i), P(F_Y, i),
+                   P(F_VX, i), P(F_VY, i), P(F_MASS, i), s.type[i], P(F_CXX, i),
+                   P(F_CXY, i), P(F_CYY, i));
     else
       std::fprintf(f,
                    "%.9g %.9g %.9g %.9g %.9g %.9g %.9g %d %.9g %.9g %.9g %.9g %.9g %.9g\n",
-                   P[F_X][i], P[F_Y][i], P[F_Z][i], P[F_VX][i], P[F_VY][i], P[F_VZ][i],
-                   P[F_MASS][i], s.type[i], P[F_CXX][i], P[F_CXY][i], P[F_CYY][i],
-                   P[F_CXZ][i], P[F_CYZ][i], P[F_CZZ][i]);
+                   P(F_X, i), P(F_Y, i), P(F_Z, i), P(F_VX, i), P(F_VY, i), P(F_VZ, i),
+                   P(F_MASS, i), s.type[i], P(F_CXX, i), P(F_CXY, i), P(F_CYY, i),
+                   P(F_CXZ, i), P(F_CYZ, i), P(F_CZZ, i));
   }
   std::fclose(f);
   return SPH_OK;

@@ -48,6 +48,28 @@ def make_system(cfg, tmpdir, precision=sph_system.FP64, noise=1e-3, seed=1, name
     return s
 
 
+def perturb(s, seed=11, pos=0.1, conf=0.05):
+    """Generic state: fluid positions jittered by U(-pos, pos) lattice spacings and
+    a random symmetric perturbation of the conformation tensor (so every stress
+    component, including the off-diagonal ones, enters the forces)."""
+    rng = np.random.default_rng(seed)
+    p = s.params
+    t = s.get("type")
+    fl = t == 0
+    dx = p.L[0] / p.Nxyz[0]
+    for a, n in enumerate(["x", "y", "z"][:p.dim]):
+        v = s.get(n)
+        v[fl] += rng.uniform(-pos, pos, fl.sum()) * dx
+        if not (a == 1 and p.wall):
+            v = np.mod(v, p.L[a])
+        s.set(n, v)
+    names = ["cxx", "cxy", "cyy"] + (["cxz", "cyz", "czz"] if p.dim == 3 else [])
+    for n in names:
+        c = s.get(n)
+        c[fl] += rng.uniform(-conf, conf, fl.sum())
+        s.set(n, c)
+
+
 def relerr(a, b):
     """Normwise relative error ||a - b|| / ||b|| (0 when both vanish)."""
     a = np.asarray(a, np.float64)

@@ -1,0 +1,62 @@
+"""The C-ABI library loads without a GPU and exports every symbol of include/sph.h."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import sph_system
+from conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "sph.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_entry_point_list():
+    assert header_functions() == sorted(sph_system.ENTRY_POINTS)
+
+
+def test_library_exports_every_header_symbol():
+    lib = ctypes.CDLL(sph_system.LIB_PATH)
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_create_rejects_bad_precision():
+    lib = sph_system.library()
+    assert not lib.sph_create(7)
+
+
+def test_params_struct_size_is_multiple_of_8():
+    assert ctypes.sizeof(sph_system.SphParams) % 8 == 0
+    names = [f[0] for f in sph_system.SphParams._fields_]
+    assert len(names) == len(set(names))
+
+
+def test_device_calls_before_initialize_fail_cleanly():
+    s = sph_system.System("fp64")
+    for call in (s.neighbours_search, s.calculate_forces, s.prepare, lambda: s.step(1)):
+        with pytest.raises(sph_system.SphError) as e:
+            call()
+        assert e.value.code == 8  # SPH_ERR_STATE
+
+
+def test_missing_input_is_an_error(tmp_path):
+    s = sph_system.System("fp32")
+    with pytest.raises(sph_system.SphError) as e:
+        s.read_input(str(tmp_path / "nope"))
+    assert e.value.code == 2  # SPH_ERR_IO
+
+
+def test_unknown_field_and_wrong_count(tmp_path):
+    import helpers
+
+    s = helpers.make_system(helpers.cfg_couette(n=16), str(tmp_path), noise=0)
+    lib = s.lib
+    buf = (ctypes.c_double * 4)()
+    assert lib.sph_get_field(s.h, b"nope", buf, 4) == 1
+    assert lib.sph_get_field(s.h, b"x", buf, 4) == 1
+    assert "entries" in lib.sph_last_error(s.h).decode()

@@ -1,12 +1,18 @@
 """Parity of the HIP path (through the C-ABI) with the FP64 oracle.
 
+States are "generic" (fluid positions jittered by 0.1 lattice spacing, random
+symmetric perturbation of c, velocity noise) unless the test says "lattice".
+
 Tolerances (normwise relative error ||gpu - oracle|| / ||oracle|| unless noted):
   cell list, sorted order, cell ranges           bit-exact (both precision forms)
-  FP64 form, after one step: dens, gradv, press, F, dc/dt, wall and colloid
+  FP64 form, generic state: dens, gradv, press, F, dc/dt, wall and colloid
       force/torque                                1e-12
-  FP64 form, positions after steps                1e-12 absolute (box length 1)
-  FP32 form (pair arithmetic in float): dens 1e-6, gradv and dc/dt 1e-4,
-      F 1e-3, wall/colloid sums 1e-3, positions 1e-9 absolute after one step
+  FP64 form, positions after 5 steps              1e-12 absolute (box length 1)
+  FP64 form, perfect-lattice start: F 1e-10 (the net force there is ~1e-3 of
+      the individual pair terms, so summation-order rounding is amplified)
+  FP32 form (pair arithmetic in float), generic state: dens 1e-6, gradv and
+      dc/dt 1e-4, F 1e-3, wall/colloid sums 1e-3; positions 1e-9 absolute
+  FP32 form, perfect-lattice start: F 1e-2
 """
 import numpy as np
 import pytest
@@ -26,8 +32,10 @@ CASES = {
 }
 
 
-def _pair(case, tmpdir, precision=sph_system.FP64):
+def _pair(case, tmpdir, precision=sph_system.FP64, generic=True):
     s = helpers.make_system(CASES[case](), tmpdir, precision)
+    if generic:
+        helpers.perturb(s)
     o = oracle.Oracle.from_system(s)
     return s, o
 
@@ -91,6 +99,16 @@ def test_prepare_fp64(case, tmp_path):
     if s.params.N_colloids:
         _check_fields(s, o, 1e-12, ["fx_colloid", "fy_colloid", "tz_colloid"] +
                       (["fz_colloid", "tx_colloid", "ty_colloid"] if d == 3 else []))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision,tol", [(sph_system.FP64, 1e-10), (sph_system.FP32, 1e-2)])
+def test_lattice_start_forces(case, precision, tol, tmp_path):
+    s, o = _pair(case, str(tmp_path), precision, generic=False)
+    s.prepare()
+    o.prepare()
+    d = s.params.dim
+    _check_fields(s, o, tol, comps("f", d))
 
 
 @pytest.mark.parametrize("case", list(CASES))
