@@ -308,73 +308,97 @@ __global__ __launch_bounds__(kBlock) void k_hash(sph_params_t P, const double *_
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int *cell, int *slot,
                                                  int *count, int *flag, int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int cx = (int)floor(x[i] / P.cell_size[0]);
-  if (cx == P.Ncells[0]) cx = cx - 1;
-  int cy;
-  if (P.wall == 0) {
-    cy = (int)floor(y[i] / P.cell_size[1]);
-    if (cy == P.Ncells[1]) cy = cy - 1;
-  } else {
-    cy = (int)floor((y[i] + P.wall_width) / P.cell_size[1]);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = -2;  // lanes past N
+  if (i < N) {
+    int cx = (int)floor(x[i] / P.cell_size[0]);
+    if (cx == P.Ncells[0]) cx = cx - 1;
+    int cy;
+    if (P.wall == 0) {
+      cy = (int)floor(y[i] / P.cell_size[1]);
+      if (cy == P.Ncells[1]) cy = cy - 1;
+    } else {
+      cy = (int)floor((y[i] + P.wall_width) / P.cell_size[1]);
+    }
+    if (P.dim == 3) {
+      int cz = (int)floor(z[i] / P.cell_size[2]);
+      if (cz == P.Ncells[2]) cz = cz - 1;
+      bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] && cz >= 0 &&
+                cz < P.Ncells[2];
+      c = ok ? cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
+    } else {
+      bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1];
+      c = ok ? cy * P.Ncells[0] + cx : -1;
+    }
+    cell[i] = c;
+    if (c == -1) atomicOr(flag, 1);
   }
-  int c;
-  if (P.dim == 3) {
-    int cz = (int)floor(z[i] / P.cell_size[2]);
-    if (cz == P.Ncells[2]) cz = cz - 1;
-    bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] && cz >= 0 &&
-              cz < P.Ncells[2];
-    c = ok ? cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
-  } else {
-    bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1];
-    c = ok ? cy * P.Ncells[0] + cx : -1;
-  }
-  if (c < 0) {
-    atomicOr(flag, 1);
-    cell[i] = -1;
-    return;
-  }
-  cell[i] = c;
-  slot[i] = atomicAdd(&count[c], 1);
+  // Particles arrive in the previous step's cell order, so neighbouring lanes
+  // mostly share a cell: one atomicAdd per run of equal cells in the wavefront,
+  // each lane's slot = run base + its position in the run.
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(c, 1, 64);
+  const bool head = lane == 0 || prev != c;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const int leader = 63 - __clzll(heads & upto);
+  const unsigned long long after = heads & ~upto;
+  const int run_end = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (lane == leader && c >= 0) base = atomicAdd(&count[c], run_end - leader);
+  base = __shfl(base, leader, 64);
+  if (c >= 0) slot[i] = base + (lane - leader);
 }
 
 // Exclusive scan of the per-cell counts into cs[0..ncells]; zeroes the counts
-// for the next hash.  One workgroup of 1024 threads, each owning a contiguous
-// chunk of cells.
+// for the next hash.  One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan_cells(int *count, int *cs, int ncells) {
+  // Tiles of 8192 cells: each thread owns 8 consecutive cells of the tile, the
+  // block scans the per-thread sums, and a running carry links the tiles.
   __shared__ int wsum[16];
+  __shared__ int tile_total;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int chunk = (ncells + 1023) / 1024;
-  const int c0 = min(t * chunk, ncells), c1 = min(c0 + chunk, ncells);
-  int local = 0;
-  for (int c = c0; c < c1; ++c) local += count[c];
-  // inclusive scan of `local` across the block
-  int incl = local;
+  int carry = 0;
+  for (int base = 0; base < ncells; base += 8192) {
+    const int c0 = base + t * 8;
+    int v[8];
+    int local = 0;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int n = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += n;
-  }
-  if (lane == 63) wsum[wid] = incl;
-  __syncthreads();
-  if (t == 0) {
-    int acc = 0;
-    for (int w = 0; w < 16; ++w) {
-      int s = wsum[w];
-      wsum[w] = acc;
-      acc += s;
+    for (int k = 0; k < 8; ++k) {
+      v[k] = c0 + k < ncells ? count[c0 + k] : 0;
+      local += v[k];
     }
+    int incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int n = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += n;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (t == 0) {
+      int acc = 0;
+      for (int w = 0; w < 16; ++w) {
+        int s = wsum[w];
+        wsum[w] = acc;
+        acc += s;
+      }
+      tile_total = acc;
+    }
+    __syncthreads();
+    int run = carry + wsum[wid] + incl - local;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (c0 + k < ncells) {
+        cs[c0 + k] = run;
+        count[c0 + k] = 0;
+      }
+      run += v[k];
+    }
+    carry += tile_total;
+    __syncthreads();
   }
-  __syncthreads();
-  int run = wsum[wid] + incl - local;
-  for (int c = c0; c < c1; ++c) {
-    int n = count[c];
-    cs[c] = run;
-    run += n;
-    count[c] = 0;
-  }
-  if (t == 1023) cs[ncells] = run;
+  if (t == 0) cs[ncells] = carry;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const int *__restrict__ cell,
@@ -421,31 +445,57 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
 }
 
 // ---------------------------------------------------------------------------
-// pair pass 1: density (+ pressure)
+// pair pass 1: density (+ pressure) and the neighbour list
 // ---------------------------------------------------------------------------
+// Scans the candidate strips once.  Every candidate inside the cut-off adds
+// W(r) to the density (self included, src/kernel_densities.cu:95-99) and, when
+// it is not the particle itself, is appended to the particle's neighbour list
+// in scan order.  The list is column-major - entry k of particle i sits at
+// nlist[k * N + i] - so a wavefront's 64 lanes write and read 256 contiguous
+// bytes per entry.  More than kmax neighbours raises bit 2 of the device flag.
+struct NList {
+  int *idx;    // kmax * N
+  int *count;  // N
+  int kmax;
+};
+
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                    const int *__restrict__ cs, int N) {
+                                                    const int *__restrict__ cs, NList L,
+                                                    int *flag, int N, int write_dens) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *__restrict__ X = S.x;
   const double *__restrict__ Y = S.y;
   const double *__restrict__ Z = S.z;
+  int *__restrict__ out = L.idx + i;
   const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
   const R rcutsq = (R)P.rcutsq, rinv = (R)P.rcut_inv, cw = (R)P.cw;
+  const int kmax = L.kmax;
   R d = R(0);
+  int cnt = 0;
   for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
     for (int k = k0; k < k1; ++k) {
       R r[3];
       pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
       R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-      if (r2 < rcutsq) d = d + sph_W<R>(sqrt(r2), rinv, cw);
+      if (r2 < rcutsq) {
+        d = d + sph_W<R>(sqrt(r2), rinv, cw);
+        if (k != i) {
+          if (cnt < kmax) out[(size_t)cnt * N] = k;
+          cnt++;
+        }
+      }
     }
   });
-  D.dens[i] = d;
-  D.press[i] = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
+  L.count[i] = cnt;
+  if (cnt > kmax) atomicOr(flag, 2);
+  if (write_dens) {
+    D.dens[i] = d;
+    D.press[i] = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
+  }
 }
 
 template <class R>
@@ -456,13 +506,54 @@ __global__ __launch_bounds__(kBlock) void k_pressure(sph_params_t P, Particles<R
   D.press[i] = (R)P.csq * (S.mass[i] * D.dens[i] - (R)P.rho0);
 }
 
+// Pi / d^2 with Pi = p I - nkT c, components xx xy yy xz yz zz
+// (src/kernel_forces.cu:64-88, 547-558).
+template <class R, int DIM>
+__device__ __forceinline__ void pi_over_d2(const R *const c[6], int k, R nkT, R pr, R dinv2,
+                                           R out[6]) {
+  out[0] = (pr - nkT * c[0][k]) * dinv2;
+  out[1] = (-nkT * c[1][k]) * dinv2;
+  out[2] = (pr - nkT * c[2][k]) * dinv2;
+  if (DIM == 3) {
+    out[3] = (-nkT * c[3][k]) * dinv2;
+    out[4] = (-nkT * c[4][k]) * dinv2;
+    out[5] = (pr - nkT * c[5][k]) * dinv2;
+  } else {
+    out[3] = out[4] = out[5] = R(0);
+  }
+}
+
+template <class R, int DIM>
+__device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, R f[3]) {
+  // kernel_ext_force (src/kernel_ext_force.cu:22-72), fluid particles only.
+  if (P.ext_force_type == 1) {
+    f[0] += m * (R)P.ext_force[0];
+    f[1] += m * (R)P.ext_force[1];
+    if (DIM == 3) f[2] += m * (R)P.ext_force[2];
+  } else if (P.ext_force_type == 2) {
+    f[0] += m * (R)P.ext_force[0] * (R)sin(P.ky * y);
+  } else if (P.ext_force_type == 3) {
+    if (y < 0.5 * P.L[1]) f[0] += m * (R)P.ext_force[0];
+    else f[0] += -(m * (R)P.ext_force[0]);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// pair pass 2: velocity gradient (+ dc/dt of fluid particles)
+// pair pass 2: velocity gradient, dc/dt, forces and external force, fused
 // ---------------------------------------------------------------------------
-template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                  Bodies B, const int *__restrict__ cs, int N,
-                                                  int store_gv, int write_dcdt) {
+// One walk over the neighbour list built by k_density.  The reference runs
+// kernel_vel_gradients (src/kernel_vel_gradients.cu:15-662) and kernel_forces
+// (src/kernel_forces.cu:15-865) as separate walks; forces need neighbour
+// densities and pressures but not the velocity gradient, so both are gathered
+// here from one evaluation of r, e, gradW and the Morris relative velocity.
+// Pair selection follows each reference kernel: the gradient skips pairs of
+// the same non-fluid type, the forces skip pairs of two non-fluid particles.
+// DO_GV / DO_F select the parts; store_gv writes the gradient tensor,
+// write_dcdt writes dc/dt (src/kernel_dcdt.cu:15-104, zero for non-fluid).
+template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
+__global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                  Bodies B, NList L, int N, int store_gv,
+                                                  int write_dcdt, int with_ext) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *__restrict__ X = S.x;
@@ -472,35 +563,46 @@ __global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S
   const R *__restrict__ VX = S.v[0];
   const R *__restrict__ VY = S.v[1];
   const R *__restrict__ VZ = S.v[2];
+  const R *__restrict__ DENS = D.dens;
+  const R *__restrict__ PRESS = D.press;
+  const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
+  const int *__restrict__ nl = L.idx + i;
   const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
   const int ti = T[i];
   const R vi[3] = {VX[i], VY[i], DIM == 3 ? VZ[i] : R(0)};
-  const R dinv = R(1) / D.dens[i];
-  int cx, cy, cz;
-  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
-  const R rcutsq = (R)P.rcutsq, rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
+  const R nkT = (R)P.nkT;
+  const R di0inv = R(1) / DENS[i];
+  const R pi0 = DO_F ? PRESS[i] : R(0);
+  R Pi0[6];
+  if (DO_F) pi_over_d2<R, DIM>(C6, i, nkT, pi0, di0inv * di0inv, Pi0);
+  const R rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
+  const R ca = (R)P.a, cb = (R)P.b;
   R g[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) g[a][b] = R(0);
-  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
-    for (int k = k0; k < k1; ++k) {
-      if (k == i) continue;
-      const int tj = T[k];
-      if (ti == tj && ti != 0) continue;
-      const double xj = X[k], yj = Y[k], zj = DIM == 3 ? Z[k] : 0.0;
-      R r[3];
-      pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
-      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-      if (!(r2 < rcutsq)) continue;
-      R rr = sqrt(r2), ri = R(1) / rr;
-      R e[3] = {r[0] * ri, r[1] * ri, r[2] * ri};
-      R gw = sph_gradW<R>(rr, rinv_c, cg);
-      const R vj[3] = {VX[k], VY[k], DIM == 3 ? VZ[k] : R(0)};
-      R v[3];
-      morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
-      R coef = -dinv * gw;
+  R f[3] = {R(0), R(0), R(0)};
+  const int cnt = min(L.count[i], L.kmax);
+  for (int k = 0; k < cnt; ++k) {
+    const int j = nl[(size_t)k * N];
+    const int tj = T[j];
+    const bool on_gv = DO_GV && (ti == 0 || tj != ti);
+    const bool on_f = DO_F && (ti == 0 || tj == 0);
+    if (!on_gv && !on_f) continue;
+    const double xj = X[j], yj = Y[j], zj = DIM == 3 ? Z[j] : 0.0;
+    R r[3];
+    pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
+    R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+    R rr = sqrt(r2), ri = R(1) / rr;
+    R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
+    R gw = sph_gradW<R>(rr, rinv_c, cg);
+    const R vj[3] = {VX[j], VY[j], DIM == 3 ? VZ[j] : R(0)};
+    R v[3];
+    morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
+    if (on_gv) {
+      // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
+      R coef = -di0inv * gw;
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
         R ce = coef * e[a];
@@ -508,7 +610,54 @@ __global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S
         for (int b = 0; b < DIM; ++b) g[a][b] += ce * v[b];
       }
     }
-  });
+    if (on_f) {
+      // Pi/d^2 of both sides, with the reference's substitutions for wall and
+      // colloid partners (src/kernel_forces.cu:533-593, 695-720)
+      R Pi[6], Pj[6], diinv, djinv;
+      if (ti == tj) {
+        diinv = di0inv;
+        djinv = R(1) / DENS[j];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
+        pi_over_d2<R, DIM>(C6, j, nkT, PRESS[j], djinv * djinv, Pj);
+      } else if (ti == 0) {
+        diinv = di0inv;
+        djinv = di0inv;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
+        pi_over_d2<R, DIM>(C6, j, nkT, pi0, djinv * djinv, Pj);
+      } else {
+        djinv = R(1) / DENS[j];
+        diinv = djinv;
+        pi_over_d2<R, DIM>(C6, j, nkT, PRESS[j], djinv * djinv, Pj);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
+      }
+      R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
+      R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]), ayy = -gw * (Pi[2] + Pj[2]);
+      if (DIM == 3) {
+        R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]), azz = -gw * (Pi[5] + Pj[5]);
+        f[0] += axx * e[0] + axy * e[1] + axz * e[2];
+        f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
+        f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
+      } else {
+        f[0] += axx * e[0] + axy * e[1];
+        f[1] += axy * e[0] + ayy * e[1];
+      }
+      R aux = gw * diinv * djinv * ri;
+      R a1 = ca * aux, a2 = cb * aux * ev;
+      f[0] += a1 * v[0] + a2 * e[0];
+      f[1] += a1 * v[1] + a2 * e[1];
+      if (DIM == 3) f[2] += a1 * v[2] + a2 * e[2];
+    }
+  }
+  if (DO_F) {
+    if (with_ext && ti == 0) ext_force<R, DIM>(P, S.mass[i], yi, f);
+    D.f[0][i] = f[0];
+    D.f[1][i] = f[1];
+    D.f[2][i] = DIM == 3 ? f[2] : R(0);
+  }
+  if (!DO_GV) return;
   if (store_gv) {
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
@@ -516,7 +665,6 @@ __global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S
       for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = g[a][b];
   }
   if (!write_dcdt) return;
-  // kernel_dcdt (src/kernel_dcdt.cu:15-104); zero for non-fluid particles.
   if (ti != 0) {
 #pragma unroll
     for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) D.dcdt[k][i] = R(0);
@@ -548,128 +696,6 @@ __global__ __launch_bounds__(kBlock) void k_gradv(sph_params_t P, Particles<R> S
     D.dcdt[4][i] = cgyz + cgzy - tinv * cyz;
     D.dcdt[5][i] = R(2) * cgzz + tinv * (R(1) - czz);
   }
-}
-
-// ---------------------------------------------------------------------------
-// pair pass 3: forces (+ external force)
-// ---------------------------------------------------------------------------
-template <class R, int DIM>
-__device__ __forceinline__ void pi_over_d2(const R *const c[6], int k, R nkT, R pr, R dinv2,
-                                           R out[6]) {
-  out[0] = (pr - nkT * c[0][k]) * dinv2;
-  out[1] = (-nkT * c[1][k]) * dinv2;
-  out[2] = (pr - nkT * c[2][k]) * dinv2;
-  if (DIM == 3) {
-    out[3] = (-nkT * c[3][k]) * dinv2;
-    out[4] = (-nkT * c[4][k]) * dinv2;
-    out[5] = (pr - nkT * c[5][k]) * dinv2;
-  } else {
-    out[3] = out[4] = out[5] = R(0);
-  }
-}
-
-template <class R, int DIM>
-__device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, R f[3]) {
-  // kernel_ext_force (src/kernel_ext_force.cu:22-72), fluid particles only.
-  if (P.ext_force_type == 1) {
-    f[0] += m * (R)P.ext_force[0];
-    f[1] += m * (R)P.ext_force[1];
-    if (DIM == 3) f[2] += m * (R)P.ext_force[2];
-  } else if (P.ext_force_type == 2) {
-    f[0] += m * (R)P.ext_force[0] * (R)sin(P.ky * y);
-  } else if (P.ext_force_type == 3) {
-    if (y < 0.5 * P.L[1]) f[0] += m * (R)P.ext_force[0];
-    else f[0] += -(m * (R)P.ext_force[0]);
-  }
-}
-
-template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_forces(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                   Bodies B, const int *__restrict__ cs, int N,
-                                                   int with_ext) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double *__restrict__ X = S.x;
-  const double *__restrict__ Y = S.y;
-  const double *__restrict__ Z = S.z;
-  const int *__restrict__ T = S.type;
-  const R *__restrict__ VX = S.v[0];
-  const R *__restrict__ VY = S.v[1];
-  const R *__restrict__ VZ = S.v[2];
-  const R *__restrict__ DENS = D.dens;
-  const R *__restrict__ PRESS = D.press;
-  const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
-  const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
-  const int ti = T[i];
-  const R vi[3] = {VX[i], VY[i], DIM == 3 ? VZ[i] : R(0)};
-  const R nkT = (R)P.nkT;
-  const R di0inv = R(1) / DENS[i];
-  const R pi0 = PRESS[i];
-  R Pi0[6];
-  pi_over_d2<R, DIM>(C6, i, nkT, pi0, di0inv * di0inv, Pi0);
-  int cx, cy, cz;
-  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
-  const R rcutsq = (R)P.rcutsq, rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
-  const R ca = (R)P.a, cb = (R)P.b;
-  R f[3] = {R(0), R(0), R(0)};
-  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
-    for (int k = k0; k < k1; ++k) {
-      if (k == i) continue;
-      const int tj = T[k];
-      if (ti != 0 && tj != 0) continue;
-      const double xj = X[k], yj = Y[k], zj = DIM == 3 ? Z[k] : 0.0;
-      R r[3];
-      pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
-      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-      if (!(r2 < rcutsq)) continue;
-      R Pi[6], Pj[6], diinv, djinv;
-      if (ti == tj) {
-        diinv = di0inv;
-        djinv = R(1) / DENS[k];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
-        pi_over_d2<R, DIM>(C6, k, nkT, PRESS[k], djinv * djinv, Pj);
-      } else if (ti == 0) {
-        diinv = di0inv;
-        djinv = di0inv;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
-        pi_over_d2<R, DIM>(C6, k, nkT, pi0, djinv * djinv, Pj);
-      } else {
-        djinv = R(1) / DENS[k];
-        diinv = djinv;
-        pi_over_d2<R, DIM>(C6, k, nkT, PRESS[k], djinv * djinv, Pj);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
-      }
-      const R vj[3] = {VX[k], VY[k], DIM == 3 ? VZ[k] : R(0)};
-      R v[3];
-      morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
-      R rr = sqrt(r2), ri = R(1) / rr;
-      R gw = sph_gradW<R>(rr, rinv_c, cg);
-      R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
-      R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
-      R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]), ayy = -gw * (Pi[2] + Pj[2]);
-      if (DIM == 3) {
-        R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]), azz = -gw * (Pi[5] + Pj[5]);
-        f[0] += axx * e[0] + axy * e[1] + axz * e[2];
-        f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
-        f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
-      } else {
-        f[0] += axx * e[0] + axy * e[1];
-        f[1] += axy * e[0] + ayy * e[1];
-      }
-      R aux = gw * diinv * djinv * ri;
-      R a1 = ca * aux, a2 = cb * aux * ev;
-      f[0] += a1 * v[0] + a2 * e[0];
-      f[1] += a1 * v[1] + a2 * e[1];
-      if (DIM == 3) f[2] += a1 * v[2] + a2 * e[2];
-    }
-  });
-  if (with_ext && ti == 0) ext_force<R, DIM>(P, S.mass[i], yi, f);
-  D.f[0][i] = f[0];
-  D.f[1][i] = f[1];
-  D.f[2][i] = DIM == 3 ? f[2] : R(0);
 }
 
 template <class R, int DIM>
@@ -1022,6 +1048,18 @@ class DeviceImpl final : public Device {
     HIPCHK(stage_src_.alloc(N_)); HIPCHK(stage_id_.alloc(N_)); HIPCHK(sorted_cell_.alloc(N_));
     HIPCHK(count_.alloc(P_.Ntotal_cells)); HIPCHK(cs_.alloc(P_.Ntotal_cells + 1));
     HIPCHK(flag_.alloc(1));
+    // Neighbour-list capacity: 1.5x the mean number of particles inside the
+    // cut-off sphere (disc in 2D) at the initial number density, plus 64.
+    {
+      double vol = P_.L[0] * (P_.L[1] + (P_.wall ? 2.0 * P_.wall_width : 0.0)) *
+                   (P_.dim == 3 ? P_.L[2] : 1.0);
+      double n0 = N_ / vol;
+      double nb = P_.dim == 3 ? n0 * 4.0 / 3.0 * M_PI * P_.rcut * P_.rcut * P_.rcut
+                              : n0 * M_PI * P_.rcut * P_.rcut;
+      kmax_ = ((int)(1.5 * nb) + 64 + 31) / 32 * 32;
+    }
+    HIPCHK(nl_idx_.alloc((size_t)kmax_ * N_));
+    HIPCHK(nl_count_.alloc(N_));
     HIPCHK(ke_part_.alloc(kKeBlocks)); HIPCHK(ke_out_.alloc(1));
     const int Nc = P_.N_colloids;
     for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(Nc));
@@ -1127,22 +1165,31 @@ class DeviceImpl final : public Device {
                                                  N_);
     HIPCHK(hipGetLastError());
     cur_ ^= 1;
+    list_valid_ = false;
     have_cells_ = true;
     id_dirty_ = true;
     return SPH_OK;
   }
 
-  int density(std::string &err) override {
+  // calculate_density: density, pressure and the neighbour list.
+  int density(std::string &err) override { return build_list(1, err); }
+
+  int build_list(int write_dens, std::string &err) {
     Mark m(this, PH_DENS);
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(),
-                                                                  cs_.p, N_);
+      k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
+          P_, parts(cur_), derived(), cs_.p, nlist(), flag_.p, N_, write_dens);
     });
     HIPCHK(hipGetLastError());
+    list_valid_ = true;
     return SPH_OK;
   }
+
+  // The list depends on positions only; it is rebuilt (without touching the
+  // densities) when positions changed since the last density pass.
+  int ensure_list(std::string &err) { return list_valid_ ? SPH_OK : build_list(0, err); }
 
   int pressure(std::string &err) override {
     k_pressure<R><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
@@ -1150,35 +1197,33 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
-  int vel_gradient(bool store_gradv, std::string &err) override {
-    // store_gradv == true: the reference's calculate_vel_gradient output only.
-    return gradv_impl(store_gradv ? 1 : 0, store_gradv ? 0 : 1, err);
-  }
-  int gradv_dcdt(bool store_gradv, std::string &err) {
-    return gradv_impl(store_gradv ? 1 : 0, 1, err);
-  }
-  int gradv_impl(int store, int dcdt, std::string &err) {
-    Mark m(this, PH_GRADV);
+  template <bool GV, bool F>
+  int pairs(int store_gv, int write_dcdt, int with_ext, int phase, std::string &err) {
+    int rc = ensure_list(err);
+    if (rc) return rc;
+    Mark m(this, phase);
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_gradv<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(),
-                                                                bodies(), cs_.p, N_, store, dcdt);
+      k_pairs<R, DIM, WALL, GV, F><<<blocks(N_), kBlock, 0, stream_>>>(
+          P_, parts(cur_), derived(), bodies(), nlist(), N_, store_gv, write_dcdt, with_ext);
     });
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
 
+  // calculate_vel_gradient: the gradient tensor only.
+  int vel_gradient(bool store_gradv, std::string &err) override {
+    return pairs<true, false>(store_gradv ? 1 : 0, 0, 0, PH_GRADV, err);
+  }
+
   int pair_forces(bool with_ext, std::string &err) override {
-    Mark m(this, PH_FORCES);
-    dispatch3([&](auto dimc, auto wallc) {
-      constexpr int DIM = decltype(dimc)::value;
-      constexpr bool WALL = decltype(wallc)::value;
-      k_forces<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), cs_.p, N_, with_ext ? 1 : 0);
-    });
-    HIPCHK(hipGetLastError());
-    return SPH_OK;
+    return pairs<false, true>(0, 0, with_ext ? 1 : 0, PH_FORCES, err);
+  }
+
+  // gradient + dc/dt + forces + external force in one walk.
+  int pairs_all(bool store_gradv, std::string &err) {
+    return pairs<true, true>(store_gradv ? 1 : 0, 1, 1, PH_FORCES, err);
   }
 
   int external_forces(std::string &err) override {
@@ -1210,6 +1255,7 @@ class DeviceImpl final : public Device {
 
   int vv1(std::string &err) override {
     Mark m(this, PH_VV1);
+    list_valid_ = false;
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
@@ -1268,8 +1314,13 @@ class DeviceImpl final : public Device {
     if (flag) {
       HIPCHK(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
       HIPCHK(hipStreamSynchronize(stream_));
-      err = "kernel cell ranges error: a particle has not a proper cell assigned";
-      return SPH_ERR_OUT_OF_GRID;
+      if (flag & 1) {
+        err = "kernel cell ranges error: a particle has not a proper cell assigned";
+        return SPH_ERR_OUT_OF_GRID;
+      }
+      err = "neighbour list overflow: a particle has more than " + std::to_string(kmax_) +
+            " neighbours";
+      return SPH_ERR_GEOMETRY;
     }
     return SPH_OK;
   }
@@ -1333,11 +1384,13 @@ class DeviceImpl final : public Device {
         for (int p = 0; p < N_; ++p) tmp[p] = hs.pf[k][id_host_[p]];
         double *dst = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
         HIPCHK(hipMemcpy(dst, tmp.data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+        list_valid_ = false;
         return SPH_OK;
       }
       return put_real(real_ptr(k), hs.pf[k], &id_host_, err);
     }
     if (name == "type") {
+      list_valid_ = false;
       std::vector<int> tmp(N_);
       for (int p = 0; p < N_; ++p) tmp[p] = hs.type[id_host_[p]];
       HIPCHK(hipMemcpy(type_[cur_].p, tmp.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
@@ -1431,6 +1484,8 @@ class DeviceImpl final : public Device {
     S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
     return S;
   }
+  NList nlist() { return NList{nl_idx_.p, nl_count_.p, kmax_}; }
+
   Derived<R> derived() {
     Derived<R> D;
     D.dens = dens_.p; D.press = press_.p;
@@ -1503,6 +1558,9 @@ class DeviceImpl final : public Device {
   DevBuf<int> type_[2], id_[2];
   DevBuf<R> dens_, press_, f_[3], dcdt_[6], gv_[9];
   DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, flag_;
+  DevBuf<int> nl_idx_, nl_count_;
+  int kmax_ = 0;
+  bool list_valid_ = false;
   DevBuf<double> ke_part_, ke_out_;
   DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_;
   DevBuf<int> clist_, cstart_, cowner_, wlist_, wstart_;
@@ -1512,8 +1570,7 @@ class DeviceImpl final : public Device {
   int forces_all(bool store_gradv, std::string &err) {
     int rc;
     if ((rc = density(err))) return rc;
-    if ((rc = gradv_dcdt(store_gradv, err))) return rc;
-    if ((rc = pair_forces(true, err))) return rc;
+    if ((rc = pairs_all(store_gradv, err))) return rc;
     if ((rc = force_walls(err))) return rc;
     return force_colloids(err);
   }

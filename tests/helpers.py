@@ -70,6 +70,41 @@ def perturb(s, seed=11, pos=0.1, conf=0.05):
         s.set(n, c)
 
 
+def sum_scales(o):
+    """Sum of |terms| of every wall and colloid reduction, from the oracle's
+    state: the scale against which a change of summation order is measured
+    (these sums cancel strongly, e.g. the torque of near-radial forces)."""
+    p = o.params
+    out = {}
+    F = [o.get(n) for n in ("fx", "fy", "fz")]
+    t = o.get("type")
+    if p.wall:
+        for a, n in enumerate(("fx_wall", "fy_wall", "fz_wall")):
+            out[n] = np.array([np.abs(F[a][t == w + 1]).sum() for w in range(p.Nwalls)])
+    if p.N_colloids:
+        X = [o.get(n) for n in ("x", "y", "z")]
+        C = [o.get(n) for n in ("coll_x", "coll_y", "coll_z")]
+        Fc = [o.get(n) for n in ("fx_colloid", "fy_colloid", "fz_colloid")]
+        sc = {n: np.zeros(p.N_colloids) for n in
+              ("fx_colloid", "fy_colloid", "fz_colloid", "tx_colloid", "ty_colloid", "tz_colloid")}
+        for k in range(p.N_colloids):
+            sh = t == 3 + k
+            r = []
+            for a in range(3):
+                d = X[a][sh] - C[a][k]
+                if not (a == 1 and p.wall):
+                    d = d - p.L[a] * np.round(d / p.L[a])
+                r.append(d if (a < p.dim) else 0 * d)
+            f = [F[a][sh] for a in range(3)]
+            for a, n in enumerate(("fx_colloid", "fy_colloid", "fz_colloid")):
+                sc[n][k] = np.abs(f[a]).sum() + abs(Fc[a][k])
+            sc["tx_colloid"][k] = np.abs(r[1] * f[2]).sum() + np.abs(r[2] * f[1]).sum()
+            sc["ty_colloid"][k] = np.abs(r[2] * f[0]).sum() + np.abs(r[0] * f[2]).sum()
+            sc["tz_colloid"][k] = np.abs(r[0] * f[1]).sum() + np.abs(r[1] * f[0]).sum()
+        out.update(sc)
+    return out
+
+
 def relerr(a, b):
     """Normwise relative error ||a - b|| / ||b|| (0 when both vanish)."""
     a = np.asarray(a, np.float64)
