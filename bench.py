@@ -155,7 +155,9 @@ def main():
             "workload": "3D walls + 16 magnetic colloids, 100^3 lattice",
             "particles": N,
             "dim": 3, "L": 1.0, "Vwall": 0.25, "tau": 1.0, "eta": 1.0, "c": 20.0,
-            "np": 100.0, "kT": 1e-3, "coll_R": 0.05, "tau_repulsion": 1.25,
+            "np": 100.0, "kT": 1e-3, "n_colloids": 16, "coll_R": 0.05,
+            "F0_repulsion": 10.0, "tau_repulsion": 1.25, "cutoff_magnetic": 0.3,
+            "F0_magnetic": 1e-3, "omega_magnetic": 6.283185307179586,
             "dt": 2e-5, "precision": "fp32-form",
         },
         "fp64": None if r64 is None else {

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -452,7 +453,8 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
 // it is not the particle itself, is appended to the particle's neighbour list
 // in scan order.  The list is column-major - entry k of particle i sits at
 // nlist[k * N + i] - so a wavefront's 64 lanes write and read 256 contiguous
-// bytes per entry.  More than kmax neighbours raises bit 2 of the device flag.
+// bytes per entry.  `count` holds the true number of neighbours even past kmax;
+// the pair pass re-scans the strips for the entries that did not fit.
 struct NList {
   int *idx;    // kmax * N
   int *count;  // N
@@ -462,7 +464,7 @@ struct NList {
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
                                                     const int *__restrict__ cs, NList L,
-                                                    int *flag, int N, int write_dens) {
+                                                    int N, int write_dens) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *__restrict__ X = S.x;
@@ -491,7 +493,6 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
     }
   });
   L.count[i] = cnt;
-  if (cnt > kmax) atomicOr(flag, 2);
   if (write_dens) {
     D.dens[i] = d;
     D.press[i] = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
@@ -552,8 +553,9 @@ __device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, 
 // write_dcdt writes dc/dt (src/kernel_dcdt.cu:15-104, zero for non-fluid).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                  Bodies B, NList L, int N, int store_gv,
-                                                  int write_dcdt, int with_ext) {
+                                                  Bodies B, NList L,
+                                                  const int *__restrict__ cs, int N,
+                                                  int store_gv, int write_dcdt, int with_ext) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *__restrict__ X = S.x;
@@ -583,13 +585,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S
 #pragma unroll
     for (int b = 0; b < 3; ++b) g[a][b] = R(0);
   R f[3] = {R(0), R(0), R(0)};
-  const int cnt = min(L.count[i], L.kmax);
-  for (int k = 0; k < cnt; ++k) {
-    const int j = nl[(size_t)k * N];
+  auto pair = [&](const int j) {
     const int tj = T[j];
     const bool on_gv = DO_GV && (ti == 0 || tj != ti);
     const bool on_f = DO_F && (ti == 0 || tj == 0);
-    if (!on_gv && !on_f) continue;
+    if (!on_gv && !on_f) return;
     const double xj = X[j], yj = Y[j], zj = DIM == 3 ? Z[j] : 0.0;
     R r[3];
     pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
@@ -650,6 +650,28 @@ __global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S
       f[1] += a1 * v[1] + a2 * e[1];
       if (DIM == 3) f[2] += a1 * v[2] + a2 * e[2];
     }
+  };
+  const int total = L.count[i];
+  const int cnt = min(total, L.kmax);
+  for (int k = 0; k < cnt; ++k) pair(nl[(size_t)k * N]);
+  if (total > cnt) {
+    // More neighbours than the list holds: re-scan the strips with the density
+    // pass's arithmetic (same hits, same order) and take the hits past the list.
+    int cx, cy, cz;
+    cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
+    const R rcutsq = (R)P.rcutsq;
+    int hit = 0;
+    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+      for (int k = k0; k < k1; ++k) {
+        R r[3];
+        pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
+        R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+        if (r2 < rcutsq && k != i) {
+          if (hit >= cnt) pair(k);
+          hit++;
+        }
+      }
+    });
   }
   if (DO_F) {
     if (with_ext && ti == 0) ext_force<R, DIM>(P, S.mass[i], yi, f);
@@ -1057,6 +1079,9 @@ class DeviceImpl final : public Device {
       double nb = P_.dim == 3 ? n0 * 4.0 / 3.0 * M_PI * P_.rcut * P_.rcut * P_.rcut
                               : n0 * M_PI * P_.rcut * P_.rcut;
       kmax_ = ((int)(1.5 * nb) + 64 + 31) / 32 * 32;
+      // SPH_NEIGHBOUR_CAPACITY overrides it (smaller lists save memory; the
+      // pair pass re-scans for particles whose neighbours do not fit).
+      if (const char *e = std::getenv("SPH_NEIGHBOUR_CAPACITY")) kmax_ = std::max(1, std::atoi(e));
     }
     HIPCHK(nl_idx_.alloc((size_t)kmax_ * N_));
     HIPCHK(nl_count_.alloc(N_));
@@ -1180,7 +1205,7 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), cs_.p, nlist(), flag_.p, N_, write_dens);
+          P_, parts(cur_), derived(), cs_.p, nlist(), N_, write_dens);
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
@@ -1206,7 +1231,8 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), nlist(), N_, store_gv, write_dcdt, with_ext);
+          P_, parts(cur_), derived(), bodies(), nlist(), cs_.p, N_, store_gv,
+          write_dcdt, with_ext);
     });
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1314,13 +1340,8 @@ class DeviceImpl final : public Device {
     if (flag) {
       HIPCHK(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
       HIPCHK(hipStreamSynchronize(stream_));
-      if (flag & 1) {
-        err = "kernel cell ranges error: a particle has not a proper cell assigned";
-        return SPH_ERR_OUT_OF_GRID;
-      }
-      err = "neighbour list overflow: a particle has more than " + std::to_string(kmax_) +
-            " neighbours";
-      return SPH_ERR_GEOMETRY;
+      err = "kernel cell ranges error: a particle has not a proper cell assigned";
+      return SPH_ERR_OUT_OF_GRID;
     }
     return SPH_OK;
   }

@@ -101,11 +101,18 @@ def sine_3d(n=64, steps=5000, dt=5e-5, **kw):
 
 
 def walls_colloids_3d(n=100, n_colloids=16, R=0.05, seed=3, steps=2000, dt=2e-5,
-                      tau_repulsion=1.25, **kw):
-    """The 1M-particle wall + magnetic colloid flow.  tau_repulsion = 1.25 instead
-    of 0.01: with 0.01 the repulsion cut-off 2R + 5R/tau is 25 box lengths and the
-    reference's colloid cell grid check (src/system_initialize.cpp:243-261)
-    rejects the input; 1.25 gives a cut-off of 0.3 = cutoff_magnetic."""
+                      tau_repulsion=1.25, F0_magnetic=1e-3, **kw):
+    """The 1M-particle wall + magnetic colloid flow.  Two parameters differ from
+    the nominal set, because the nominal values cannot run:
+      * tau_repulsion = 1.25 instead of 0.01: with 0.01 the repulsion cut-off
+        2R + 5R/tau is 25 box lengths and the colloid cell grid check
+        (src/system_initialize.cpp:243-261) rejects the input; 1.25 gives a
+        cut-off of 0.3 = cutoff_magnetic.
+      * F0_magnetic = 1e-3 instead of 1: with 1 the dipole force F0/r^4 at the
+        seeded spacing (>= 0.135) is ~150 on a colloid of mass 5e-4, which drives
+        the colloids far past the sound speed c = 20 within ~20 steps (the
+        dynamics diverge, in the reference as here); 1e-3 gives a Stokes
+        terminal velocity ~0.16 (Mach ~0.01)."""
     cfg = dict(BASE)
     L = [1.0, 1.0, 1.0]
     cfg.update(N=[n, n, n], L=L, dim=3, dt=dt, Nsteps=steps, tau=1.0, ext_force=[0.0, 0.0, 0.0],
@@ -113,7 +120,7 @@ def walls_colloids_3d(n=100, n_colloids=16, R=0.05, seed=3, steps=2000, dt=2e-5,
                coll_rho=1.0, coll_move=0,
                coll_x=place_colloids(n_colloids, R, L, 3, True, seed),
                coll_repulsion_cuton=2 * R, F0_repulsion=10.0, tau_repulsion=tau_repulsion,
-               cutoff_magnetic=0.3, F0_magnetic=1.0, omega_magnetic=2 * math.pi)
+               cutoff_magnetic=0.3, F0_magnetic=F0_magnetic, omega_magnetic=2 * math.pi)
     cfg.update(kw)
     return cfg
 

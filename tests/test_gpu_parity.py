@@ -251,3 +251,22 @@ def test_out_of_grid_particle_is_reported(tmp_path):
     with pytest.raises(sph_system.SphError) as e:
         s.neighbours_search()
     assert e.value.code == 6
+
+
+def test_small_neighbour_list_capacity_gives_identical_results(tmp_path, monkeypatch):
+    """With a list capacity of 8 every particle overflows; the pair pass then
+    re-scans its strips for the remaining neighbours in the same order, so the
+    results are bitwise those of the default capacity."""
+    outs = []
+    for cap in (None, "8"):
+        if cap:
+            monkeypatch.setenv("SPH_NEIGHBOUR_CAPACITY", cap)
+        s = helpers.make_system(helpers.cfg_colloids_3d(n=24), str(tmp_path), name=f"i{cap}")
+        helpers.perturb(s)
+        s.prepare()
+        s.step(2)
+        s.synchronize()
+        outs.append([s.get(n) for n in ("fx", "fy", "fz", "dcdt_xy", "vx", "tz_colloid")])
+        s.close()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
