@@ -170,6 +170,19 @@ int sph_run(sph_handle h, const char *out_dir);
 /* Total kinetic energy sum_i 0.5 m_i |v_i|^2 over all particles, FP64
  * deterministic reduction (src/kernel_calculate_macro_vars.cu:14-40). */
 int sph_kinetic_energy(sph_handle h, double *ke);
+/* Rheology observables of the current state (the sweep outputs; the reference
+ * has no counterpart - it only writes the raw wall/colloid files):
+ *   out[0] kinetic energy (as sph_kinetic_energy)
+ *   out[1] first normal-stress difference of the polymer, nkT <cxx - cyy>_fluid
+ *   out[2] polymer shear stress, nkT <cxy>_fluid
+ *   out[3] bottom-wall shear stress Fx_wall[0] / A, out[4] top wall, with
+ *          A = L0 (2D) or L0*L2 (3D); NaN without walls
+ *   out[5] colloid chain angle: nematic mean 0.5 atan2(sum sin 2phi, sum cos 2phi)
+ *          of the in-plane angle phi of colloid pairs closer than cutoff_magnetic
+ *          (NaN when there is no such pair)
+ *   out[6] field angle omega_magnetic (t0 + dt step), out[7] phase lag out[6] - out[5];
+ *          all three angles reduced modulo pi into (-pi/2, pi/2]. */
+int sph_observables(sph_handle h, double out[8]);
 /* class_system::print_output (src/system_print_output.cpp:14-80): writes
  * micro-<step>.dat, walls-<step>.dat, colloids-<step>.dat and appends to
  * macro.dat in `out_dir` when step is a multiple of the matching freq_*. */

@@ -52,6 +52,8 @@ def library():
     lib.orc_step.argtypes = [vp, i32]
     lib.orc_kinetic_energy.restype = ctypes.c_double
     lib.orc_kinetic_energy.argtypes = [vp]
+    lib.orc_observables.restype = i32
+    lib.orc_observables.argtypes = [vp, vp]
     lib.orc_num_threads.restype = i32
     lib.orc_num_threads.argtypes = []
     lib.orc_kernel_W.restype = ctypes.c_double
@@ -198,6 +200,11 @@ class Oracle:
 
     def kinetic_energy(self):
         return self.lib.orc_kinetic_energy(self.h)
+
+    def observables(self):
+        out = np.zeros(8, np.float64)
+        self.lib.orc_observables(self.h, _ptr(out))
+        return out
 
     def density_brute_force(self):
         out = np.zeros(self.params.N, np.float64)

@@ -962,6 +962,42 @@ double orc_kinetic_energy(orc_system *s) {
   return ke;
 }
 
+int orc_observables(orc_system *s, double out[8]) {
+  const sph_params_t &p = s->p;
+  const double nan = std::nan("");
+  out[0] = orc_kinetic_energy(s);
+  double n1 = 0.0, sxy = 0.0, nf = 0.0;
+  for (int i = 0; i < p.N; ++i) {
+    if (s->type[i] != 0) continue;
+    n1 += s->f[CXX][i] - s->f[CYY][i];
+    sxy += s->f[CXY][i];
+    nf += 1.0;
+  }
+  out[1] = nf > 0 ? p.nkT * n1 / nf : nan;
+  out[2] = nf > 0 ? p.nkT * sxy / nf : nan;
+  const double area = p.dim == 3 ? p.L[0] * p.L[2] : p.L[0];
+  out[3] = p.wall ? s->fwall[0][0] / area : nan;
+  out[4] = p.wall ? s->fwall[0][1] / area : nan;
+  double ss = 0.0, cc = 0.0, np_ = 0.0;
+  for (int i = 0; i < p.N_colloids; ++i)
+    for (int j = i + 1; j < p.N_colloids; ++j) {
+      double dx = mimg(s->cf[CX][j] - s->cf[CX][i], p.L[0], 0.5 * p.L[0]);
+      double dy = s->cf[CY][j] - s->cf[CY][i];
+      if (p.wall == 0) dy = mimg(dy, p.L[1], 0.5 * p.L[1]);
+      double dz = p.dim == 3 ? mimg(s->cf[CZ][j] - s->cf[CZ][i], p.L[2], 0.5 * p.L[2]) : 0.0;
+      if (dx * dx + dy * dy + dz * dz > p.r0_magnet_sq) continue;
+      double phi = std::atan2(dy, dx);
+      ss += std::sin(2.0 * phi);
+      cc += std::cos(2.0 * phi);
+      np_ += 1.0;
+    }
+  auto half_turn = [](double a) { return a - M_PI * std::ceil(a / M_PI - 0.5); };
+  out[5] = np_ > 0 ? half_turn(0.5 * std::atan2(ss, cc)) : nan;
+  out[6] = half_turn(p.omega_magnet * (p.t0 + p.dt * (double)s->step));
+  out[7] = half_turn(out[6] - out[5]);
+  return SPH_OK;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

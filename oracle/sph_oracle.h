@@ -47,6 +47,8 @@ int orc_prepare(orc_system *s);
 /* nsteps loop bodies of src/main.cu:358-459 from the internal step counter */
 int orc_step(orc_system *s, int nsteps);
 double orc_kinetic_energy(orc_system *s);
+/* The sweep observables, defined as for sph_observables in sph.h. */
+int orc_observables(orc_system *s, double out[8]);
 int orc_num_threads(void);
 
 /* Stand-alone pieces for closed-form checks. */

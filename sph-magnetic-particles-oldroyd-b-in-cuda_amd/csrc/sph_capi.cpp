@@ -271,6 +271,31 @@ int sph_kinetic_energy(sph_handle h, double *ke) {
   return h->dev->kinetic_energy(ke, h->err);
 }
 
+int sph_observables(sph_handle h, double out[8]) {
+  if (!h || !out) return SPH_ERR_ARG;
+  int rc = ensure_device(h);
+  if (rc) return rc;
+  if ((rc = h->dev->kinetic_energy(&out[0], h->err))) return rc;
+  double s[6];
+  if ((rc = h->dev->observable_sums(s, h->err))) return rc;
+  const sph_params_t &p = h->hs.p;
+  const double nan = std::nan("");
+  out[1] = s[2] > 0 ? p.nkT * s[0] / s[2] : nan;
+  out[2] = s[2] > 0 ? p.nkT * s[1] / s[2] : nan;
+  out[3] = out[4] = nan;
+  if (p.wall) {
+    if ((rc = pull(h, {"fx_wall"}))) return rc;
+    const double area = p.dim == 3 ? p.L[0] * p.L[2] : p.L[0];
+    out[3] = h->hs.fwall[0][0] / area;
+    out[4] = h->hs.fwall[0][1] / area;
+  }
+  auto half_turn = [](double a) { return a - M_PI * std::ceil(a / M_PI - 0.5); };
+  out[5] = s[5] > 0 ? half_turn(0.5 * std::atan2(s[3], s[4])) : nan;
+  out[6] = half_turn(p.omega_magnet * (p.t0 + p.dt * (double)h->step));
+  out[7] = half_turn(out[6] - out[5]);
+  return SPH_OK;
+}
+
 int sph_print_output(sph_handle h, int step, const char *out_dir) {
   if (!h || !out_dir) return SPH_ERR_ARG;
   if (!h->hs.initialized) { h->err = "the system is not initialized"; return SPH_ERR_STATE; }

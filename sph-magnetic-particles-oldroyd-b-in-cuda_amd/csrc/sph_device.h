@@ -32,6 +32,9 @@ class Device {
   virtual int vv2(std::string &err) = 0;
   virtual void set_magnetic_moment(const double m[3]) = 0;
   virtual int kinetic_energy(double *ke, std::string &err) = 0;
+  // [0] sum(cxx-cyy) [1] sum(cxy) over fluid, [2] fluid count, [3] sum sin 2phi,
+  // [4] sum cos 2phi, [5] count of colloid pairs within the magnetic cut-off.
+  virtual int observable_sums(double out[6], std::string &err) = 0;
   virtual int synchronize(std::string &err) = 0;
   // Checks the device error flag (out-of-grid particle); resets it.
   virtual int check_flags(std::string &err) = 0;

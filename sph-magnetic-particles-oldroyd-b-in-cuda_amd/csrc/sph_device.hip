@@ -41,6 +41,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kRedBlock = 1024;
 constexpr int kKeBlocks = 1024;
+constexpr int kTailBlocks = 64;
 
 #define HIPCHK(expr)                                                          \
   do {                                                                        \
@@ -66,6 +67,8 @@ struct Particles {  // per-particle state that moves with the reorder
 template <class R>
 struct Derived {  // per-particle fields recomputed every step
   R *dens, *press;
+  R *dinv;   // 1 / dens
+  R *pd[6];  // (p I - nkT c) / dens^2, components xx xy yy xz yz zz
   R *f[3];
   R *dcdt[6];
   R *gv[9];  // xx xy xz yx yy yz zx zy zz ; gv_ab = dv_b/dx_a
@@ -134,34 +137,39 @@ __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int 
   // A particle outside the grid (already flagged by k_hash) has no neighbours;
   // this keeps every cs[] index in bounds.
   if (cx < 0 || cx > Nx || cy < -1 || cy > Ny || (DIM == 3 && (cz < 0 || cz > Nz))) return;
-#pragma unroll
-  for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
-    int z = cz + dz;
+  // Cells cx-1..cx+1 of a row as x-ranges [a0, a1) then [b0, b1): one range
+  // inside the row, two at the periodic seam, visited in the order x-1, x, x+1.
+  // (cx == Nx happens for x == L, whose neighbours are Nx-1, 0, 1.)
+  int a0, a1, b0 = 0, b1 = 0;
+  if (cx >= 1 && cx <= Nx - 2) {
+    a0 = cx - 1; a1 = cx + 2;
+  } else if (cx == Nx - 1) {
+    a0 = Nx - 2; a1 = Nx; b1 = 1;
+  } else {
+    a0 = Nx - 1; a1 = Nx; b1 = 2;
+  }
+  // One loop over (row, part) with a single call site keeps the caller's body
+  // instantiated once.
+  constexpr int kRows = DIM == 3 ? 9 : 3;
+#pragma unroll 1
+  for (int s = 0; s < 2 * kRows; ++s) {
+    const int part = s & 1;
+    if (part && b1 == 0) continue;
+    const int row = s >> 1;
+    int y = cy + row % 3 - 1;
+    int z = DIM == 3 ? cz + row / 3 - 1 : 0;
     if (DIM == 3) {
       if (z < 0) z += Nz;
       if (z >= Nz) z -= Nz;
     }
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      int y = cy + dy;
-      if (!WALL) {
-        if (y < 0) y += Ny;
-        if (y >= Ny) y -= Ny;
-      } else if (y < 0 || y >= Ny) {
-        continue;
-      }
-      const int base = (z * Ny + y) * Nx;
-      if (cx >= 1 && cx <= Nx - 2) {
-        fn(cs[base + cx - 1], cs[base + cx + 2]);
-      } else {
-        for (int dx = -1; dx <= 1; ++dx) {
-          int x = cx + dx;
-          if (x < 0) x += Nx;
-          if (x >= Nx) x -= Nx;
-          fn(cs[base + x], cs[base + x + 1]);
-        }
-      }
+    if (!WALL) {
+      if (y < 0) y += Ny;
+      if (y >= Ny) y -= Ny;
+    } else if (y < 0 || y >= Ny) {
+      continue;
     }
+    const int base = (z * Ny + y) * Nx;
+    fn(cs[base + (part ? b0 : a0)], cs[base + (part ? b1 : a1)]);
   }
 }
 
@@ -456,21 +464,46 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
 // bytes per entry.  `count` holds the true number of neighbours even past kmax;
 // the pair pass re-scans the strips for the entries that did not fit.
 struct NList {
-  int *idx;    // kmax * N
-  int *count;  // N
-  int kmax;
+  int4 *idx;   // (kmax / 4) * N groups of four entries
+  int *count;  // N: true neighbour count (may exceed kmax)
+  int *queue;  // particles whose neighbours did not all fit (k_pairs -> k_pairs_tail)
+  int *qlen;   // number of queued particles
+  int kmax;    // multiple of 4
 };
+
+// Pressure p = c^2 (m d - rho0) (src/kernel_pressures.cu:21) and the per-particle
+// factors the force pass gathers: 1/d and Pi/d^2 with Pi = p I - nkT c, in the
+// reference's arithmetic (src/kernel_forces.cu:64-88, 547-558).
+template <class R>
+__device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Particles<R> &S,
+                                               const Derived<R> &D, int i, R d) {
+  const R p = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
+  const R dinv = R(1) / d;
+  const R dinv2 = dinv * dinv;
+  const R nkT = (R)P.nkT;
+  D.press[i] = p;
+  D.dinv[i] = dinv;
+  D.pd[0][i] = (p - nkT * S.c[0][i]) * dinv2;
+  D.pd[1][i] = (-nkT * S.c[1][i]) * dinv2;
+  D.pd[2][i] = (p - nkT * S.c[2][i]) * dinv2;
+  if (P.dim == 3) {
+    D.pd[3][i] = (-nkT * S.c[3][i]) * dinv2;
+    D.pd[4][i] = (-nkT * S.c[4][i]) * dinv2;
+    D.pd[5][i] = (p - nkT * S.c[5][i]) * dinv2;
+  }
+}
 
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
                                                     const int *__restrict__ cs, NList L,
                                                     int N, int write_dens) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
   if (i >= N) return;
   const double *__restrict__ X = S.x;
   const double *__restrict__ Y = S.y;
   const double *__restrict__ Z = S.z;
-  int *__restrict__ out = L.idx + i;
+  int4 *__restrict__ out = L.idx + i;
   const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
@@ -478,6 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
   const int kmax = L.kmax;
   R d = R(0);
   int cnt = 0;
+  int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
   for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
     for (int k = k0; k < k1; ++k) {
       R r[3];
@@ -486,16 +520,22 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
       if (r2 < rcutsq) {
         d = d + sph_W<R>(sqrt(r2), rinv, cw);
         if (k != i) {
-          if (cnt < kmax) out[(size_t)cnt * N] = k;
+          buf = make_int4(buf.y, buf.z, buf.w, k);
           cnt++;
+          if ((cnt & 3) == 0 && cnt <= kmax) out[(size_t)((cnt >> 2) - 1) * N] = buf;
         }
       }
     }
   });
+  const int tail = cnt & 3;
+  if (tail && cnt < kmax) {
+    for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
+    out[(size_t)(cnt >> 2) * N] = buf;
+  }
   L.count[i] = cnt;
   if (write_dens) {
     D.dens[i] = d;
-    D.press[i] = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
+    pressure_terms<R>(P, S, D, i, d);
   }
 }
 
@@ -504,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure(sph_params_t P, Particles<R
                                                      int N) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  D.press[i] = (R)P.csq * (S.mass[i] * D.dens[i] - (R)P.rho0);
+  pressure_terms<R>(P, S, D, i, D.dens[i]);
 }
 
 // Pi / d^2 with Pi = p I - nkT c, components xx xy yy xz yz zz
@@ -539,159 +579,145 @@ __device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, 
   }
 }
 
-// ---------------------------------------------------------------------------
-// pair pass 2: velocity gradient, dc/dt, forces and external force, fused
-// ---------------------------------------------------------------------------
-// One walk over the neighbour list built by k_density.  The reference runs
-// kernel_vel_gradients (src/kernel_vel_gradients.cu:15-662) and kernel_forces
-// (src/kernel_forces.cu:15-865) as separate walks; forces need neighbour
-// densities and pressures but not the velocity gradient, so both are gathered
-// here from one evaluation of r, e, gradW and the Morris relative velocity.
-// Pair selection follows each reference kernel: the gradient skips pairs of
-// the same non-fluid type, the forces skip pairs of two non-fluid particles.
-// DO_GV / DO_F select the parts; store_gv writes the gradient tensor,
-// write_dcdt writes dc/dt (src/kernel_dcdt.cu:15-104, zero for non-fluid).
+// Per-particle accumulators of the fused pair pass.
+template <class R>
+struct PairAcc {
+  R g[3][3];  // velocity gradient sum
+  R f[3];     // force sum
+};
+
+// Everything about particle i that the pair body reads.
+template <class R>
+struct PairSelf {
+  double x, y, z;
+  int t;
+  R v[3];
+  R dinv, p;
+  R pd[6];
+};
+
+// One neighbour j of i: velocity-gradient and force contributions with the
+// reference's pair selection, Morris velocities and Pi/d^2 substitutions
+// (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                  Bodies B, NList L,
-                                                  const int *__restrict__ cs, int N,
-                                                  int store_gv, int write_dcdt, int with_ext) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double *__restrict__ X = S.x;
-  const double *__restrict__ Y = S.y;
-  const double *__restrict__ Z = S.z;
-  const int *__restrict__ T = S.type;
-  const R *__restrict__ VX = S.v[0];
-  const R *__restrict__ VY = S.v[1];
-  const R *__restrict__ VZ = S.v[2];
-  const R *__restrict__ DENS = D.dens;
-  const R *__restrict__ PRESS = D.press;
-  const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
-  const int *__restrict__ nl = L.idx + i;
-  const double xi = X[i], yi = Y[i], zi = DIM == 3 ? Z[i] : 0.0;
-  const int ti = T[i];
-  const R vi[3] = {VX[i], VY[i], DIM == 3 ? VZ[i] : R(0)};
-  const R nkT = (R)P.nkT;
-  const R di0inv = R(1) / DENS[i];
-  const R pi0 = DO_F ? PRESS[i] : R(0);
-  R Pi0[6];
-  if (DO_F) pi_over_d2<R, DIM>(C6, i, nkT, pi0, di0inv * di0inv, Pi0);
-  const R rinv_c = (R)P.rcut_inv, cg = (R)P.c_gradw;
-  const R ca = (R)P.a, cb = (R)P.b;
-  R g[3][3];
+__device__ __forceinline__ void pair_body(const sph_params_t &P, const Particles<R> &S,
+                                          const Derived<R> &D, const Bodies &B,
+                                          const PairSelf<R> &me, int j, PairAcc<R> &acc) {
+  const int tj = S.type[j];
+  const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
+  const bool on_f = DO_F && (me.t == 0 || tj == 0);
+  if (!on_gv && !on_f) return;
+  const double xj = S.x[j], yj = S.y[j], zj = DIM == 3 ? S.z[j] : 0.0;
+  R r[3];
+  pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, xj, yj, zj, r);
+  const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+  const R rr = sqrt(r2), ri = R(1) / rr;
+  const R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
+  const R gw = sph_gradW<R>(rr, (R)P.rcut_inv, (R)P.c_gradw);
+  const R vj[3] = {S.v[0][j], S.v[1][j], DIM == 3 ? S.v[2][j] : R(0)};
+  R v[3];
+  morris_vij<R, DIM, WALL>(P, B, me.t, tj, me.x, me.y, me.z, xj, yj, zj, me.v, vj, r, v);
+  if (on_gv) {
+    // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
+    const R coef = -me.dinv * gw;
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < DIM; ++a) {
+      const R ce = coef * e[a];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) g[a][b] = R(0);
-  R f[3] = {R(0), R(0), R(0)};
-  auto pair = [&](const int j) {
-    const int tj = T[j];
-    const bool on_gv = DO_GV && (ti == 0 || tj != ti);
-    const bool on_f = DO_F && (ti == 0 || tj == 0);
-    if (!on_gv && !on_f) return;
-    const double xj = X[j], yj = Y[j], zj = DIM == 3 ? Z[j] : 0.0;
-    R r[3];
-    pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj, yj, zj, r);
-    R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-    R rr = sqrt(r2), ri = R(1) / rr;
-    R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
-    R gw = sph_gradW<R>(rr, rinv_c, cg);
-    const R vj[3] = {VX[j], VY[j], DIM == 3 ? VZ[j] : R(0)};
-    R v[3];
-    morris_vij<R, DIM, WALL>(P, B, ti, tj, xi, yi, zi, xj, yj, zj, vi, vj, r, v);
-    if (on_gv) {
-      // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
-      R coef = -di0inv * gw;
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) {
-        R ce = coef * e[a];
-#pragma unroll
-        for (int b = 0; b < DIM; ++b) g[a][b] += ce * v[b];
-      }
+      for (int b = 0; b < DIM; ++b) acc.g[a][b] += ce * v[b];
     }
-    if (on_f) {
-      // Pi/d^2 of both sides, with the reference's substitutions for wall and
-      // colloid partners (src/kernel_forces.cu:533-593, 695-720)
-      R Pi[6], Pj[6], diinv, djinv;
-      if (ti == tj) {
-        diinv = di0inv;
-        djinv = R(1) / DENS[j];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
-        pi_over_d2<R, DIM>(C6, j, nkT, PRESS[j], djinv * djinv, Pj);
-      } else if (ti == 0) {
-        diinv = di0inv;
-        djinv = di0inv;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pi0[q];
-        pi_over_d2<R, DIM>(C6, j, nkT, pi0, djinv * djinv, Pj);
-      } else {
-        djinv = R(1) / DENS[j];
-        diinv = djinv;
-        pi_over_d2<R, DIM>(C6, j, nkT, PRESS[j], djinv * djinv, Pj);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
-      }
-      R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
-      R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]), ayy = -gw * (Pi[2] + Pj[2]);
-      if (DIM == 3) {
-        R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]), azz = -gw * (Pi[5] + Pj[5]);
-        f[0] += axx * e[0] + axy * e[1] + axz * e[2];
-        f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
-        f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
-      } else {
-        f[0] += axx * e[0] + axy * e[1];
-        f[1] += axy * e[0] + ayy * e[1];
-      }
-      R aux = gw * diinv * djinv * ri;
-      R a1 = ca * aux, a2 = cb * aux * ev;
-      f[0] += a1 * v[0] + a2 * e[0];
-      f[1] += a1 * v[1] + a2 * e[1];
-      if (DIM == 3) f[2] += a1 * v[2] + a2 * e[2];
-    }
-  };
-  const int total = L.count[i];
-  const int cnt = min(total, L.kmax);
-  for (int k = 0; k < cnt; ++k) pair(nl[(size_t)k * N]);
-  if (total > cnt) {
-    // More neighbours than the list holds: re-scan the strips with the density
-    // pass's arithmetic (same hits, same order) and take the hits past the list.
-    int cx, cy, cz;
-    cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
-    const R rcutsq = (R)P.rcutsq;
-    int hit = 0;
-    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
-      for (int k = k0; k < k1; ++k) {
-        R r[3];
-        pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
-        R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-        if (r2 < rcutsq && k != i) {
-          if (hit >= cnt) pair(k);
-          hit++;
-        }
-      }
-    });
   }
+  if (on_f) {
+    R Pi[6], Pj[6], diinv, djinv;
+    if (me.t == tj) {  // fluid - fluid
+      diinv = me.dinv;
+      djinv = D.dinv[j];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
+#pragma unroll
+      for (int q = 0; q < (DIM == 3 ? 6 : 3); ++q) Pj[q] = D.pd[q][j];
+      if (DIM == 2) Pj[3] = Pj[4] = Pj[5] = R(0);
+    } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure
+      diinv = me.dinv;
+      djinv = me.dinv;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
+      const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
+      pi_over_d2<R, DIM>(C6, j, (R)P.nkT, me.p, djinv * djinv, Pj);
+    } else {  // wall/colloid i: both sides use fluid j's d, p and c
+      djinv = D.dinv[j];
+      diinv = djinv;
+#pragma unroll
+      for (int q = 0; q < (DIM == 3 ? 6 : 3); ++q) Pj[q] = D.pd[q][j];
+      if (DIM == 2) Pj[3] = Pj[4] = Pj[5] = R(0);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
+    }
+    const R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
+    const R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]);
+    const R ayy = -gw * (Pi[2] + Pj[2]);
+    if (DIM == 3) {
+      const R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]);
+      const R azz = -gw * (Pi[5] + Pj[5]);
+      acc.f[0] += axx * e[0] + axy * e[1] + axz * e[2];
+      acc.f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
+      acc.f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
+    } else {
+      acc.f[0] += axx * e[0] + axy * e[1];
+      acc.f[1] += axy * e[0] + ayy * e[1];
+    }
+    const R aux = gw * diinv * djinv * ri;
+    const R a1 = (R)P.a * aux, a2 = (R)P.b * aux * ev;
+    acc.f[0] += a1 * v[0] + a2 * e[0];
+    acc.f[1] += a1 * v[1] + a2 * e[1];
+    if (DIM == 3) acc.f[2] += a1 * v[2] + a2 * e[2];
+  }
+}
+
+template <class R, int DIM>
+__device__ __forceinline__ void load_self(const sph_params_t &P, const Particles<R> &S,
+                                          const Derived<R> &D, int i, bool forces,
+                                          PairSelf<R> &me) {
+  me.x = S.x[i];
+  me.y = S.y[i];
+  me.z = DIM == 3 ? S.z[i] : 0.0;
+  me.t = S.type[i];
+  me.v[0] = S.v[0][i];
+  me.v[1] = S.v[1][i];
+  me.v[2] = DIM == 3 ? S.v[2][i] : R(0);
+  me.dinv = D.dinv[i];
+  me.p = forces ? D.press[i] : R(0);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) me.pd[q] = (forces && (DIM == 3 || q < 3)) ? D.pd[q][i] : R(0);
+}
+
+// Writes the results of particle i: force (+ external force), gradient tensor
+// (store_gv), dc/dt of fluid particles (write_dcdt; src/kernel_dcdt.cu:76-102).
+template <class R, int DIM, bool DO_GV, bool DO_F>
+__device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particles<R> &S,
+                                            const Derived<R> &D, int i, const PairSelf<R> &me,
+                                            PairAcc<R> &acc, int store_gv, int write_dcdt,
+                                            int with_ext) {
   if (DO_F) {
-    if (with_ext && ti == 0) ext_force<R, DIM>(P, S.mass[i], yi, f);
-    D.f[0][i] = f[0];
-    D.f[1][i] = f[1];
-    D.f[2][i] = DIM == 3 ? f[2] : R(0);
+    if (with_ext && me.t == 0) ext_force<R, DIM>(P, S.mass[i], me.y, acc.f);
+    D.f[0][i] = acc.f[0];
+    D.f[1][i] = acc.f[1];
+    D.f[2][i] = DIM == 3 ? acc.f[2] : R(0);
   }
   if (!DO_GV) return;
   if (store_gv) {
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
 #pragma unroll
-      for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = g[a][b];
+      for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = acc.g[a][b];
   }
   if (!write_dcdt) return;
-  if (ti != 0) {
+  if (me.t != 0) {
 #pragma unroll
     for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) D.dcdt[k][i] = R(0);
     return;
   }
+  const R(&g)[3][3] = acc.g;
   const R tinv = R(1) / (R)P.tau;
   const R cxx = S.c[0][i], cxy = S.c[1][i], cyy = S.c[2][i];
   if (DIM == 2) {
@@ -717,6 +743,98 @@ __global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S
     D.dcdt[3][i] = cgxz + cgzx - tinv * cxz;
     D.dcdt[4][i] = cgyz + cgzy - tinv * cyz;
     D.dcdt[5][i] = R(2) * cgzz + tinv * (R(1) - czz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pair pass 2: velocity gradient, dc/dt, forces and external force, fused
+// ---------------------------------------------------------------------------
+// One walk over the neighbour list built by k_density.  The reference runs
+// kernel_vel_gradients (src/kernel_vel_gradients.cu:15-662) and kernel_forces
+// (src/kernel_forces.cu:15-865) as separate walks; forces need neighbour
+// densities and pressures but not the velocity gradient, so both are gathered
+// here from one evaluation of r, e, gradW and the Morris relative velocity.
+// DO_GV / DO_F select the parts.  A particle with more neighbours than the list
+// holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
+template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
+__global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
+                                                  Bodies B, NList L, int N, int store_gv,
+                                                  int write_dcdt, int with_ext) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  PairSelf<R> me;
+  load_self<R, DIM>(P, S, D, i, DO_F, me);
+  PairAcc<R> acc;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    acc.f[a] = R(0);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) acc.g[a][b] = R(0);
+  }
+  const int total = L.count[i];
+  const int cnt = min(total, L.kmax);
+  const int4 *__restrict__ nl = L.idx + i;
+#pragma unroll 1
+  for (int k = 0; k < cnt; k += 4) {
+    int4 q = nl[(size_t)(k >> 2) * N];
+    const int m = min(4, cnt - k);
+#pragma unroll 1
+    for (int u = 0; u < m; ++u) {
+      pair_body<R, DIM, WALL, DO_GV, DO_F>(P, S, D, B, me, q.x, acc);
+      q = make_int4(q.y, q.z, q.w, 0);
+    }
+  }
+  if (total > cnt) {
+    // park partial sums; k_pairs_tail continues from them
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      D.f[a][i] = acc.f[a];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) D.gv[a * 3 + b][i] = acc.g[a][b];
+    }
+    L.queue[atomicAdd(L.qlen, 1)] = i;
+    return;
+  }
+  pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+}
+
+// Particles whose neighbours did not all fit in the list: re-scan the strips
+// with the density pass's arithmetic (same hits, same order), skip the hits the
+// list held and add the rest, then finish as k_pairs would have.
+template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
+__global__ __launch_bounds__(kBlock) void k_pairs_tail(sph_params_t P, Particles<R> S,
+                                                       Derived<R> D, Bodies B, NList L,
+                                                       const int *__restrict__ cs, int store_gv,
+                                                       int write_dcdt, int with_ext) {
+  const int n = *L.qlen;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int i = L.queue[q];
+    PairSelf<R> me;
+    load_self<R, DIM>(P, S, D, i, DO_F, me);
+    PairAcc<R> acc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      acc.f[a] = D.f[a][i];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc.g[a][b] = D.gv[a * 3 + b][i];
+    }
+    int cx, cy, cz;
+    cell_coords<DIM, WALL>(P, me.x, me.y, me.z, cx, cy, cz);
+    const R rcutsq = (R)P.rcutsq;
+    const int skip = L.kmax;
+    int hit = 0;
+    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+      for (int k = k0; k < k1; ++k) {
+        R r[3];
+        pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, r);
+        R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+        if (r2 < rcutsq && k != i) {
+          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, S, D, B, me, k, acc);
+          hit++;
+        }
+      }
+    });
+    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
   }
 }
 
@@ -1025,6 +1143,54 @@ __global__ __launch_bounds__(kBlock) void k_ke_final(const double *part, int n, 
 }
 
 // ---------------------------------------------------------------------------
+// rheology observables (sweep outputs; no counterpart kernel in the reference)
+// ---------------------------------------------------------------------------
+// Per block: sum over fluid particles of (cxx - cyy), cxy and the count.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_polymer_partial(Particles<R> S, double *part, int N) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+    if (S.type[i] != 0) continue;
+    acc[0] += (double)S.c[0][i] - (double)S.c[2][i];
+    acc[1] += (double)S.c[1][i];
+    acc[2] += 1.0;
+  }
+  block_sum<3, kBlock>(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = acc[k];
+}
+
+// One block: finishes the polymer sums and the colloid chain orientation,
+// the nematic mean of the in-plane angle of every colloid pair closer than the
+// magnetic cut-off.  out: [0] sum(cxx-cyy) [1] sum(cxy) [2] fluid count
+// [3] sum sin(2 phi) [4] sum cos(2 phi) [5] pair count.
+__global__ __launch_bounds__(kBlock) void k_observables_final(sph_params_t P, Bodies B,
+                                                              const double *part, int nparts,
+                                                              double *out) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
+    for (int k = 0; k < 3; ++k) acc[k] += part[3 * b + k];
+  const int nc = B.n_colloids;
+  const long long npairs = (long long)nc * nc;
+  for (long long q = threadIdx.x; q < npairs; q += kBlock) {
+    const int i = (int)(q / nc), j = (int)(q % nc);
+    if (j <= i) continue;
+    double dx = mimg(B.s[C_X][j] - B.s[C_X][i], P.L[0], 0.5 * P.L[0]);
+    double dy = B.s[C_Y][j] - B.s[C_Y][i];
+    if (P.wall == 0) dy = mimg(dy, P.L[1], 0.5 * P.L[1]);
+    double dz = P.dim == 3 ? mimg(B.s[C_Z][j] - B.s[C_Z][i], P.L[2], 0.5 * P.L[2]) : 0.0;
+    if (dx * dx + dy * dy + dz * dz > P.r0_magnet_sq) continue;
+    const double phi = atan2(dy, dx);
+    acc[3] += sin(2.0 * phi);
+    acc[4] += cos(2.0 * phi);
+    acc[5] += 1.0;
+  }
+  block_sum<6, kBlock>(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) out[k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
 // owner of the device state
 // ---------------------------------------------------------------------------
 template <class T>
@@ -1083,9 +1249,16 @@ class DeviceImpl final : public Device {
       // pair pass re-scans for particles whose neighbours do not fit).
       if (const char *e = std::getenv("SPH_NEIGHBOUR_CAPACITY")) kmax_ = std::max(1, std::atoi(e));
     }
-    HIPCHK(nl_idx_.alloc((size_t)kmax_ * N_));
+    kmax_ = (kmax_ + 3) / 4 * 4;
+    HIPCHK(nl_idx_.alloc((size_t)(kmax_ / 4) * N_));
     HIPCHK(nl_count_.alloc(N_));
+    HIPCHK(nl_queue_.alloc(N_));
+    HIPCHK(nl_qlen_.alloc(1));
+    HIPCHK(hipMemset(nl_qlen_.p, 0, sizeof(int)));
+    HIPCHK(dinv_.alloc(N_));
+    for (int a = 0; a < 6; ++a) HIPCHK(pd_[a].alloc(N_));
     HIPCHK(ke_part_.alloc(kKeBlocks)); HIPCHK(ke_out_.alloc(1));
+    HIPCHK(obs_part_.alloc(3 * kKeBlocks)); HIPCHK(obs_out_.alloc(6));
     const int Nc = P_.N_colloids;
     for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(Nc));
     HIPCHK(clist_.alloc(P_.Nlist_colloids)); HIPCHK(cstart_.alloc(Nc + 1));
@@ -1231,8 +1404,9 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), nlist(), cs_.p, N_, store_gv,
-          write_dcdt, with_ext);
+          P_, parts(cur_), derived(), bodies(), nlist(), N_, store_gv, write_dcdt, with_ext);
+      k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
+          P_, parts(cur_), derived(), bodies(), nlist(), cs_.p, store_gv, write_dcdt, with_ext);
     });
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1313,6 +1487,16 @@ class DeviceImpl final : public Device {
     mag_[0] = m[0];
     mag_[1] = m[1];
     mag_[2] = m[2];
+  }
+
+  int observable_sums(double out[6], std::string &err) override {
+    k_polymer_partial<R><<<kKeBlocks, kBlock, 0, stream_>>>(parts(cur_), obs_part_.p, N_);
+    k_observables_final<<<1, kBlock, 0, stream_>>>(P_, bodies(), obs_part_.p, kKeBlocks,
+                                                   obs_out_.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, obs_out_.p, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return SPH_OK;
   }
 
   int kinetic_energy(double *ke, std::string &err) override {
@@ -1505,11 +1689,12 @@ class DeviceImpl final : public Device {
     S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
     return S;
   }
-  NList nlist() { return NList{nl_idx_.p, nl_count_.p, kmax_}; }
+  NList nlist() { return NList{nl_idx_.p, nl_count_.p, nl_queue_.p, nl_qlen_.p, kmax_}; }
 
   Derived<R> derived() {
     Derived<R> D;
-    D.dens = dens_.p; D.press = press_.p;
+    D.dens = dens_.p; D.press = press_.p; D.dinv = dinv_.p;
+    for (int a = 0; a < 6; ++a) D.pd[a] = pd_[a].p;
     for (int a = 0; a < 3; ++a) D.f[a] = f_[a].p;
     for (int a = 0; a < 6; ++a) D.dcdt[a] = dcdt_[a].p;
     for (int a = 0; a < 9; ++a) D.gv[a] = gv_[a].p;
@@ -1577,12 +1762,13 @@ class DeviceImpl final : public Device {
   DevBuf<double> x_[2], y_[2], z_[2];
   DevBuf<R> v_[2][3], c_[2][6], mass_[2];
   DevBuf<int> type_[2], id_[2];
-  DevBuf<R> dens_, press_, f_[3], dcdt_[6], gv_[9];
+  DevBuf<R> dens_, press_, dinv_, pd_[6], f_[3], dcdt_[6], gv_[9];
   DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, flag_;
-  DevBuf<int> nl_idx_, nl_count_;
+  DevBuf<int4> nl_idx_;
+  DevBuf<int> nl_count_, nl_queue_, nl_qlen_;
   int kmax_ = 0;
   bool list_valid_ = false;
-  DevBuf<double> ke_part_, ke_out_;
+  DevBuf<double> ke_part_, ke_out_, obs_part_, obs_out_;
   DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_;
   DevBuf<int> clist_, cstart_, cowner_, wlist_, wstart_;
 

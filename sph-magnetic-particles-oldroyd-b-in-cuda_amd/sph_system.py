@@ -35,8 +35,10 @@ ENTRY_POINTS = [
     "sph_calculate_force_colloids", "sph_calculate_forces", "sph_V_Verlet_step1",
     "sph_V_Verlet_step2", "sph_set_step", "sph_prepare", "sph_step", "sph_synchronize",
     "sph_current_step", "sph_run", "sph_kinetic_energy", "sph_print_output",
-    "sph_write_restart", "sph_set_timing", "sph_get_timing",
+    "sph_write_restart", "sph_set_timing", "sph_get_timing", "sph_observables",
 ]
+OBSERVABLES = ["kinetic_energy", "polymer_N1", "polymer_shear", "wall_shear_bottom",
+               "wall_shear_top", "chain_angle", "field_angle", "phase_lag"]
 
 PARTICLE_FIELDS = [
     "x", "y", "z", "vx", "vy", "vz", "mass", "dens", "press", "fx", "fy", "fz",
@@ -148,6 +150,8 @@ def library(path=None):
     lib.sph_run.argtypes = [vp, cp]
     lib.sph_write_restart.restype = i32
     lib.sph_write_restart.argtypes = [vp, cp]
+    lib.sph_observables.restype = i32
+    lib.sph_observables.argtypes = [vp, vp]
     lib.sph_get_timing.restype = i32
     lib.sph_get_timing.argtypes = [vp, vp, vp]
     if path is None:
@@ -315,6 +319,12 @@ class System:
         ke = ctypes.c_double(0.0)
         self._check(self.lib.sph_kinetic_energy(self.h, ctypes.byref(ke)), "kinetic_energy")
         return ke.value
+
+    def observables(self):
+        """Dict of the sweep observables (see sph_observables in sph.h)."""
+        out = np.zeros(8, np.float64)
+        self._check(self.lib.sph_observables(self.h, _ptr(out)), "observables")
+        return dict(zip(OBSERVABLES, out.tolist()))
 
     def print_output(self, step, out_dir):
         self._check(self.lib.sph_print_output(self.h, int(step), os.fsencode(out_dir)),

@@ -270,3 +270,16 @@ def test_small_neighbour_list_capacity_gives_identical_results(tmp_path, monkeyp
         s.close()
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["colloids3d", "colloids2d", "sine3d"])
+def test_observables(case, tmp_path):
+    s, o = _pair(case, str(tmp_path))
+    s.prepare()
+    o.prepare()
+    s.step(3)
+    o.step(3)
+    a = np.array(list(s.observables().values()))
+    b = o.observables()
+    both_nan = np.isnan(a) & np.isnan(b)
+    np.testing.assert_allclose(a[~both_nan], b[~both_nan], rtol=1e-10, atol=1e-12)

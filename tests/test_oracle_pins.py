@@ -437,3 +437,37 @@ def test_wall_force_is_sum_over_wall_particles(tmp_path):
         sel = t == w + 1
         assert fxw[w] == pytest.approx(o.get("fx")[sel].sum(), rel=1e-12)
         assert fyw[w] == pytest.approx(o.get("fy")[sel].sum(), rel=1e-12)
+
+
+# ---------------------------------------------------------------- observables
+def test_observables_closed_forms(tmp_path):
+    s, o = _two_colloids(tmp_path, 2, 0.25)
+    p = s.params
+    # two colloids 0.2 apart along the direction at 150 degrees
+    phi = math.radians(150.0)
+    X = np.array([0.5, 0.5 + 0.2 * math.cos(phi)])
+    Y = np.array([0.5, 0.5 + 0.2 * math.sin(phi)])
+    o.set("coll_x", X)
+    o.set("coll_y", Y)
+    t = o.get("type")
+    rng = np.random.default_rng(9)
+    cxx, cyy, cxy = (1 + rng.uniform(0, 1, s.N)), (1 + rng.uniform(0, 1, s.N)), rng.normal(size=s.N)
+    for n, v in (("cxx", cxx), ("cyy", cyy), ("cxy", cxy)):
+        o.set(n, v)
+    o.set("fx", rng.normal(size=s.N))
+    o.neighbours_search()
+    o.calculate_force_walls()
+    o.set_step(11)
+    out = o.observables()
+    fl = t == 0
+    assert out[1] == pytest.approx(p.nkT * np.mean(cxx[fl] - cyy[fl]), rel=1e-12)
+    assert out[2] == pytest.approx(p.nkT * np.mean(cxy[fl]), rel=1e-12)
+    assert out[3] == pytest.approx(o.get("fx")[t == 1].sum() / p.L[0], rel=1e-12)
+    assert out[5] == pytest.approx(phi - math.pi, abs=1e-12)     # 150 deg -> -30 deg mod pi
+    fa = p.omega_magnet * (p.t0 + p.dt * 11)
+    fa -= math.pi * math.ceil(fa / math.pi - 0.5)
+    assert out[6] == pytest.approx(fa, abs=1e-12)
+    lag = out[6] - out[5]
+    lag -= math.pi * math.ceil(lag / math.pi - 0.5)
+    assert out[7] == pytest.approx(lag, abs=1e-12)
+    assert -math.pi / 2 < out[7] <= math.pi / 2
