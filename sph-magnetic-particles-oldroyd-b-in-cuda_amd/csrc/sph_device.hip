@@ -92,6 +92,38 @@ __device__ __forceinline__ double mimg(double d, double L, double h) {
   return d;
 }
 
+// Square root and reciprocal of pair separations.  Double: correctly rounded,
+// like the reference's sqrt and division.  Float (the FP32 form): the hardware
+// v_sqrt_f32 / v_rcp_f32 (1 ulp) unless SPH_FP32_IEEE_MATH is defined.
+template <class R>
+__device__ __forceinline__ R pair_sqrt(R x) { return sqrt(x); }
+template <class R>
+__device__ __forceinline__ R pair_rcp(R x) { return R(1) / x; }
+#ifndef SPH_FP32_IEEE_MATH
+template <>
+__device__ __forceinline__ float pair_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <>
+__device__ __forceinline__ float pair_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+#endif
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  The pair kernels map workgroup b to
+// tile xcd_tile(b) so that XCD k walks the k-th contiguous eighth of the
+// cell-sorted particles and its L2 keeps the neighbour data of that slab.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = b % kXcds;
+  return x * q + min(x, r) + b / kXcds;
+}
+
+// Minimum resident wavefronts per SIMD requested for the fused pair kernel
+// (caps its registers).
+#ifndef SPH_PAIRS_MIN_WAVES
+#define SPH_PAIRS_MIN_WAVES 1
+#endif
+
 // kernel_W (src/kernel_W.cu:13-59) as selects: cw*((a^5 - 6 b^5) + 15 c^5).
 template <class R>
 __device__ __forceinline__ R sph_W(R r, R rcut_inv, R cw) {
@@ -497,7 +529,7 @@ template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
                                                     const int *__restrict__ cs, NList L,
                                                     int N, int write_dens) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
   if (i >= N) return;
   const double *__restrict__ X = S.x;
@@ -518,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
       pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
       R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
       if (r2 < rcutsq) {
-        d = d + sph_W<R>(sqrt(r2), rinv, cw);
+        d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
         if (k != i) {
           buf = make_int4(buf.y, buf.z, buf.w, k);
           cnt++;
@@ -611,7 +643,7 @@ __device__ __forceinline__ void pair_body(const sph_params_t &P, const Particles
   R r[3];
   pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, xj, yj, zj, r);
   const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-  const R rr = sqrt(r2), ri = R(1) / rr;
+  const R rr = pair_sqrt(r2), ri = pair_rcp(rr);
   const R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
   const R gw = sph_gradW<R>(rr, (R)P.rcut_inv, (R)P.c_gradw);
   const R vj[3] = {S.v[0][j], S.v[1][j], DIM == 3 ? S.v[2][j] : R(0)};
@@ -757,10 +789,10 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // DO_GV / DO_F select the parts.  A particle with more neighbours than the list
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__global__ __launch_bounds__(kBlock) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
+__global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
                                                   Bodies B, NList L, int N, int store_gv,
                                                   int write_dcdt, int with_ext) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= N) return;
   PairSelf<R> me;
   load_self<R, DIM>(P, S, D, i, DO_F, me);
