@@ -68,7 +68,6 @@ template <class R>
 struct Derived {  // per-particle fields recomputed every step
   R *dens, *press;
   R *dinv;   // 1 / dens
-  R *pd[6];  // (p I - nkT c) / dens^2, components xx xy yy xz yz zz
   R *f[3];
   R *dcdt[6];
   R *gv[9];  // xx xy xz yx yy yz zx zy zz ; gv_ab = dv_b/dx_a
@@ -158,10 +157,14 @@ __device__ __forceinline__ void cell_coords(const sph_params_t &P, double x, dou
   cz = DIM == 3 ? (int)floor(z / P.cell_size[2]) : 0;
 }
 
-// Calls fn(k0, k1) for each contiguous sorted range covering the 9 (2D: 3)
-// neighbour strips of cell (cx, cy, cz).  Interior strips are one range of
-// three consecutive cells; at the periodic x seam the three cells are visited
-// one by one.  Rows outside the wall-bounded y range are skipped.
+// Calls fn(k0, k1, sx, sy, sz) for each contiguous sorted range covering the 9
+// (2D: 3) neighbour strips of cell (cx, cy, cz), in the reference's cell order
+// (dz outer, dy, then x-1, x, x+1).  Interior strips are one range of three
+// consecutive cells; at the periodic x seam a strip splits in two.  Rows outside
+// the wall-bounded y range are skipped.  (sx, sy, sz) is the periodic shift -
+// 0 or +-L per axis - of the range's cells: for every pair closer than the
+// cut-off, (x_i - x_j) + s equals the minimum image the reference computes
+// pair by pair (cells are at least rcut wide and the box at least 3 cells).
 template <int DIM, bool WALL, class Fn>
 __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int *__restrict__ cs,
                                                int cx, int cy, int cz, Fn &&fn) {
@@ -169,16 +172,19 @@ __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int 
   // A particle outside the grid (already flagged by k_hash) has no neighbours;
   // this keeps every cs[] index in bounds.
   if (cx < 0 || cx > Nx || cy < -1 || cy > Ny || (DIM == 3 && (cz < 0 || cz > Nz))) return;
-  // Cells cx-1..cx+1 of a row as x-ranges [a0, a1) then [b0, b1): one range
-  // inside the row, two at the periodic seam, visited in the order x-1, x, x+1.
-  // (cx == Nx happens for x == L, whose neighbours are Nx-1, 0, 1.)
+  // Cells cx-1..cx+1 of a row as x-ranges [a0, a1) then [b0, b1), with shifts
+  // sa, sb.  (cx == Nx happens for x == L, whose neighbours are Nx-1, 0, 1.)
+  const double Lx = P.L[0];
   int a0, a1, b0 = 0, b1 = 0;
+  double sa = 0.0, sb = 0.0;
   if (cx >= 1 && cx <= Nx - 2) {
     a0 = cx - 1; a1 = cx + 2;
   } else if (cx == Nx - 1) {
-    a0 = Nx - 2; a1 = Nx; b1 = 1;
+    a0 = Nx - 2; a1 = Nx; b1 = 1; sb = -Lx;
+  } else if (cx == 0) {
+    a0 = Nx - 1; a1 = Nx; b1 = 2; sa = Lx;
   } else {
-    a0 = Nx - 1; a1 = Nx; b1 = 2;
+    a0 = Nx - 1; a1 = Nx; b1 = 2; sb = -Lx;
   }
   // One loop over (row, part) with a single call site keeps the caller's body
   // instantiated once.
@@ -190,19 +196,29 @@ __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int 
     const int row = s >> 1;
     int y = cy + row % 3 - 1;
     int z = DIM == 3 ? cz + row / 3 - 1 : 0;
+    double sy = 0.0, sz = 0.0;
     if (DIM == 3) {
-      if (z < 0) z += Nz;
-      if (z >= Nz) z -= Nz;
+      if (z < 0) { z += Nz; sz = P.L[2]; }
+      if (z >= Nz) { z -= Nz; sz = -P.L[2]; }
     }
     if (!WALL) {
-      if (y < 0) y += Ny;
-      if (y >= Ny) y -= Ny;
+      if (y < 0) { y += Ny; sy = P.L[1]; }
+      if (y >= Ny) { y -= Ny; sy = -P.L[1]; }
     } else if (y < 0 || y >= Ny) {
       continue;
     }
     const int base = (z * Ny + y) * Nx;
-    fn(cs[base + (part ? b0 : a0)], cs[base + (part ? b1 : a1)]);
+    fn(cs[base + (part ? b0 : a0)], cs[base + (part ? b1 : a1)], part ? sb : sa, sy, sz);
   }
+}
+
+// r_ij of a candidate inside a strip segment: (x_i - x_j) + s in double, rounded to R.
+template <class R, int DIM>
+__device__ __forceinline__ void seg_sep(double xi, double yi, double zi, double xj, double yj,
+                                        double zj, double sx, double sy, double sz, R r[3]) {
+  r[0] = (R)((xi - xj) + sx);
+  r[1] = (R)((yi - yj) + sy);
+  r[2] = DIM == 3 ? (R)((zi - zj) + sz) : R(0);
 }
 
 // r_ij = x_i - x_j with the minimum image (x, z, and y without walls), computed
@@ -503,32 +519,76 @@ struct NList {
   int kmax;    // multiple of 4
 };
 
-// Pressure p = c^2 (m d - rho0) (src/kernel_pressures.cu:21) and the per-particle
-// factors the force pass gathers: 1/d and Pi/d^2 with Pi = p I - nkT c, in the
-// reference's arithmetic (src/kernel_forces.cu:64-88, 547-558).
+// What the pair pass gathers about a neighbour j, packed into one record so a
+// neighbour costs one 64-byte line (FP32 form; two for FP64) instead of one
+// line per structure-of-arrays field: position, velocity, w = 1/d for fluid or
+// -type for wall/colloid particles, and q = Pi/d^2 (Pi = p I - nkT c) for fluid
+// or the raw conformation tensor c for wall/colloid particles (the only parts
+// the reference's force and gradient kernels read of such a partner,
+// src/kernel_forces.cu:196-233, 320-336).
+template <class R>
+struct PairRec;
+template <>
+struct alignas(64) PairRec<float> {
+  double x, y, z;
+  float v[3];
+  float w;
+  float q[6];
+};
+template <>
+struct alignas(128) PairRec<double> {
+  double x, y, z;
+  double v[3];
+  double w;
+  double q[6];
+  double pad[3];
+};
+static_assert(sizeof(PairRec<float>) == 64, "FP32 pair record must be one 64-byte line");
+static_assert(sizeof(PairRec<double>) == 128, "FP64 pair record must be two lines");
+
+// Pressure p = c^2 (m d - rho0) (src/kernel_pressures.cu:21), 1/d, and the pair
+// record of particle i, with Pi/d^2 in the reference's arithmetic
+// (src/kernel_forces.cu:64-88, 547-558).
 template <class R>
 __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Particles<R> &S,
-                                               const Derived<R> &D, int i, R d) {
+                                               const Derived<R> &D, PairRec<R> *rec, int i,
+                                               R d) {
   const R p = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
   const R dinv = R(1) / d;
   const R dinv2 = dinv * dinv;
   const R nkT = (R)P.nkT;
   D.press[i] = p;
   D.dinv[i] = dinv;
-  D.pd[0][i] = (p - nkT * S.c[0][i]) * dinv2;
-  D.pd[1][i] = (-nkT * S.c[1][i]) * dinv2;
-  D.pd[2][i] = (p - nkT * S.c[2][i]) * dinv2;
-  if (P.dim == 3) {
-    D.pd[3][i] = (-nkT * S.c[3][i]) * dinv2;
-    D.pd[4][i] = (-nkT * S.c[4][i]) * dinv2;
-    D.pd[5][i] = (p - nkT * S.c[5][i]) * dinv2;
+  const int t = S.type[i];
+  const bool d3 = P.dim == 3;
+  R c[6];
+  for (int k = 0; k < 6; ++k) c[k] = (d3 || k < 3) ? S.c[k][i] : R(0);
+  PairRec<R> r;
+  r.x = S.x[i];
+  r.y = S.y[i];
+  r.z = d3 ? S.z[i] : 0.0;
+  r.v[0] = S.v[0][i];
+  r.v[1] = S.v[1][i];
+  r.v[2] = d3 ? S.v[2][i] : R(0);
+  if (t == 0) {
+    r.w = dinv;
+    r.q[0] = (p - nkT * c[0]) * dinv2;
+    r.q[1] = (-nkT * c[1]) * dinv2;
+    r.q[2] = (p - nkT * c[2]) * dinv2;
+    r.q[3] = d3 ? (-nkT * c[3]) * dinv2 : R(0);
+    r.q[4] = d3 ? (-nkT * c[4]) * dinv2 : R(0);
+    r.q[5] = d3 ? (p - nkT * c[5]) * dinv2 : R(0);
+  } else {
+    r.w = -(R)t;
+    for (int k = 0; k < 6; ++k) r.q[k] = c[k];
   }
+  rec[i] = r;
 }
 
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                    const int *__restrict__ cs, NList L,
-                                                    int N, int write_dens) {
+                                                    PairRec<R> *rec, const int *__restrict__ cs,
+                                                    NList L, int N, int write_dens) {
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
   if (i >= N) return;
@@ -544,10 +604,11 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
   R d = R(0);
   int cnt = 0;
   int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
-  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
+                                                    double sz) {
     for (int k = k0; k < k1; ++k) {
       R r[3];
-      pair_sep<R, DIM, WALL>(P, xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, r);
+      seg_sep<R, DIM>(xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, sx, sy, sz, r);
       R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
       if (r2 < rcutsq) {
         d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
@@ -567,16 +628,16 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
   L.count[i] = cnt;
   if (write_dens) {
     D.dens[i] = d;
-    pressure_terms<R>(P, S, D, i, d);
+    pressure_terms<R>(P, S, D, rec, i, d);
   }
 }
 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_pressure(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                     int N) {
+                                                     PairRec<R> *rec, int N) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  pressure_terms<R>(P, S, D, i, D.dens[i]);
+  pressure_terms<R>(P, S, D, rec, i, D.dens[i]);
 }
 
 // Pi / d^2 with Pi = p I - nkT c, components xx xy yy xz yz zz
@@ -632,21 +693,22 @@ struct PairSelf {
 // reference's pair selection, Morris velocities and Pi/d^2 substitutions
 // (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__device__ __forceinline__ void pair_body(const sph_params_t &P, const Particles<R> &S,
-                                          const Derived<R> &D, const Bodies &B,
-                                          const PairSelf<R> &me, int j, PairAcc<R> &acc) {
-  const int tj = S.type[j];
+__device__ __forceinline__ void pair_body(const sph_params_t &P, const PairRec<R> *__restrict__ rec,
+                                          const Bodies &B, const PairSelf<R> &me, int j,
+                                          PairAcc<R> &acc) {
+  const PairRec<R> rj = rec[j];
+  const int tj = rj.w < R(0) ? (int)(-rj.w) : 0;
   const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
   const bool on_f = DO_F && (me.t == 0 || tj == 0);
   if (!on_gv && !on_f) return;
-  const double xj = S.x[j], yj = S.y[j], zj = DIM == 3 ? S.z[j] : 0.0;
+  const double xj = rj.x, yj = rj.y, zj = rj.z;
   R r[3];
   pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, xj, yj, zj, r);
   const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
   const R rr = pair_sqrt(r2), ri = pair_rcp(rr);
   const R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
   const R gw = sph_gradW<R>(rr, (R)P.rcut_inv, (R)P.c_gradw);
-  const R vj[3] = {S.v[0][j], S.v[1][j], DIM == 3 ? S.v[2][j] : R(0)};
+  const R vj[3] = {rj.v[0], rj.v[1], rj.v[2]};
   R v[3];
   morris_vij<R, DIM, WALL>(P, B, me.t, tj, me.x, me.y, me.z, xj, yj, zj, me.v, vj, r, v);
   if (on_gv) {
@@ -663,27 +725,29 @@ __device__ __forceinline__ void pair_body(const sph_params_t &P, const Particles
     R Pi[6], Pj[6], diinv, djinv;
     if (me.t == tj) {  // fluid - fluid
       diinv = me.dinv;
-      djinv = D.dinv[j];
+      djinv = rj.w;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
-#pragma unroll
-      for (int q = 0; q < (DIM == 3 ? 6 : 3); ++q) Pj[q] = D.pd[q][j];
-      if (DIM == 2) Pj[3] = Pj[4] = Pj[5] = R(0);
-    } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure
+      for (int q = 0; q < 6; ++q) {
+        Pi[q] = me.pd[q];
+        Pj[q] = rj.q[q];
+      }
+    } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure, keeps its c
       diinv = me.dinv;
       djinv = me.dinv;
+      const R nkT = (R)P.nkT, d2 = djinv * djinv;
 #pragma unroll
       for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
-      const R *const C6[6] = {S.c[0], S.c[1], S.c[2], S.c[3], S.c[4], S.c[5]};
-      pi_over_d2<R, DIM>(C6, j, (R)P.nkT, me.p, djinv * djinv, Pj);
+      Pj[0] = (me.p - nkT * rj.q[0]) * d2;
+      Pj[1] = (-nkT * rj.q[1]) * d2;
+      Pj[2] = (me.p - nkT * rj.q[2]) * d2;
+      Pj[3] = DIM == 3 ? (-nkT * rj.q[3]) * d2 : R(0);
+      Pj[4] = DIM == 3 ? (-nkT * rj.q[4]) * d2 : R(0);
+      Pj[5] = DIM == 3 ? (me.p - nkT * rj.q[5]) * d2 : R(0);
     } else {  // wall/colloid i: both sides use fluid j's d, p and c
-      djinv = D.dinv[j];
+      djinv = rj.w;
       diinv = djinv;
 #pragma unroll
-      for (int q = 0; q < (DIM == 3 ? 6 : 3); ++q) Pj[q] = D.pd[q][j];
-      if (DIM == 2) Pj[3] = Pj[4] = Pj[5] = R(0);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) Pi[q] = Pj[q];
+      for (int q = 0; q < 6; ++q) Pi[q] = Pj[q] = rj.q[q];
     }
     const R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
     const R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]);
@@ -708,8 +772,8 @@ __device__ __forceinline__ void pair_body(const sph_params_t &P, const Particles
 
 template <class R, int DIM>
 __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles<R> &S,
-                                          const Derived<R> &D, int i, bool forces,
-                                          PairSelf<R> &me) {
+                                          const Derived<R> &D, const PairRec<R> *rec, int i,
+                                          bool forces, PairSelf<R> &me) {
   me.x = S.x[i];
   me.y = S.y[i];
   me.z = DIM == 3 ? S.z[i] : 0.0;
@@ -719,8 +783,10 @@ __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles
   me.v[2] = DIM == 3 ? S.v[2][i] : R(0);
   me.dinv = D.dinv[i];
   me.p = forces ? D.press[i] : R(0);
+  // Pi/d^2 of a fluid particle (only fluid particles use their own)
+  const bool own = forces && me.t == 0;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) me.pd[q] = (forces && (DIM == 3 || q < 3)) ? D.pd[q][i] : R(0);
+  for (int q = 0; q < 6; ++q) me.pd[q] = own ? rec[i].q[q] : R(0);
 }
 
 // Writes the results of particle i: force (+ external force), gradient tensor
@@ -790,12 +856,13 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                  Bodies B, NList L, int N, int store_gv,
-                                                  int write_dcdt, int with_ext) {
+                                                  Bodies B, const PairRec<R> *__restrict__ rec,
+                                                  NList L, int N, int store_gv, int write_dcdt,
+                                                  int with_ext) {
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= N) return;
   PairSelf<R> me;
-  load_self<R, DIM>(P, S, D, i, DO_F, me);
+  load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
   PairAcc<R> acc;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -812,7 +879,7 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_param
     const int m = min(4, cnt - k);
 #pragma unroll 1
     for (int u = 0; u < m; ++u) {
-      pair_body<R, DIM, WALL, DO_GV, DO_F>(P, S, D, B, me, q.x, acc);
+      pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, q.x, acc);
       q = make_int4(q.y, q.z, q.w, 0);
     }
   }
@@ -835,14 +902,15 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_param
 // list held and add the rest, then finish as k_pairs would have.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock) void k_pairs_tail(sph_params_t P, Particles<R> S,
-                                                       Derived<R> D, Bodies B, NList L,
+                                                       Derived<R> D, Bodies B,
+                                                       const PairRec<R> *__restrict__ rec, NList L,
                                                        const int *__restrict__ cs, int store_gv,
                                                        int write_dcdt, int with_ext) {
   const int n = *L.qlen;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const int i = L.queue[q];
     PairSelf<R> me;
-    load_self<R, DIM>(P, S, D, i, DO_F, me);
+    load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
     PairAcc<R> acc;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -855,13 +923,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(sph_params_t P, Particles
     const R rcutsq = (R)P.rcutsq;
     const int skip = L.kmax;
     int hit = 0;
-    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1) {
+    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
+                                                      double sz) {
       for (int k = k0; k < k1; ++k) {
         R r[3];
-        pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, r);
+        seg_sep<R, DIM>(me.x, me.y, me.z, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, sx, sy, sz, r);
         R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
         if (r2 < rcutsq && k != i) {
-          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, S, D, B, me, k, acc);
+          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, k, acc);
           hit++;
         }
       }
@@ -1288,7 +1357,7 @@ class DeviceImpl final : public Device {
     HIPCHK(nl_qlen_.alloc(1));
     HIPCHK(hipMemset(nl_qlen_.p, 0, sizeof(int)));
     HIPCHK(dinv_.alloc(N_));
-    for (int a = 0; a < 6; ++a) HIPCHK(pd_[a].alloc(N_));
+    HIPCHK(rec_.alloc(N_));
     HIPCHK(ke_part_.alloc(kKeBlocks)); HIPCHK(ke_out_.alloc(1));
     HIPCHK(obs_part_.alloc(3 * kKeBlocks)); HIPCHK(obs_out_.alloc(6));
     const int Nc = P_.N_colloids;
@@ -1396,6 +1465,7 @@ class DeviceImpl final : public Device {
     HIPCHK(hipGetLastError());
     cur_ ^= 1;
     list_valid_ = false;
+    rec_valid_ = false;
     have_cells_ = true;
     id_dirty_ = true;
     return SPH_OK;
@@ -1410,10 +1480,11 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), cs_.p, nlist(), N_, write_dens);
+          P_, parts(cur_), derived(), rec_.p, cs_.p, nlist(), N_, write_dens);
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
+    if (write_dens) rec_valid_ = true;
     return SPH_OK;
   }
 
@@ -1421,9 +1492,11 @@ class DeviceImpl final : public Device {
   // densities) when positions changed since the last density pass.
   int ensure_list(std::string &err) { return list_valid_ ? SPH_OK : build_list(0, err); }
 
+  // calculate_pressure: pressure, 1/d and the pair records from the current densities.
   int pressure(std::string &err) override {
-    k_pressure<R><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+    k_pressure<R><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), rec_.p, N_);
     HIPCHK(hipGetLastError());
+    rec_valid_ = true;
     return SPH_OK;
   }
 
@@ -1431,14 +1504,19 @@ class DeviceImpl final : public Device {
   int pairs(int store_gv, int write_dcdt, int with_ext, int phase, std::string &err) {
     int rc = ensure_list(err);
     if (rc) return rc;
+    // Records hold positions, velocities and stresses: refresh them if any of
+    // those changed since the last density/pressure pass.
+    if (!rec_valid_ && (rc = pressure(err))) return rc;
     Mark m(this, phase);
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), nlist(), N_, store_gv, write_dcdt, with_ext);
+          P_, parts(cur_), derived(), bodies(), rec_.p, nlist(), N_, store_gv, write_dcdt,
+          with_ext);
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), nlist(), cs_.p, store_gv, write_dcdt, with_ext);
+          P_, parts(cur_), derived(), bodies(), rec_.p, nlist(), cs_.p, store_gv, write_dcdt,
+          with_ext);
     });
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1488,6 +1566,7 @@ class DeviceImpl final : public Device {
   int vv1(std::string &err) override {
     Mark m(this, PH_VV1);
     list_valid_ = false;
+    rec_valid_ = false;
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
@@ -1505,6 +1584,7 @@ class DeviceImpl final : public Device {
 
   int vv2(std::string &err) override {
     Mark m(this, PH_VV2);
+    rec_valid_ = false;
     if (P_.dim == 3)
       k_vv2<R, 3><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
     else
@@ -1612,6 +1692,7 @@ class DeviceImpl final : public Device {
 
   int upload_field(const HostState &hs, const std::string &name, std::string &err) override {
     HIPCHK(hipStreamSynchronize(stream_));
+    rec_valid_ = false;
     int rc = refresh_ids(err);
     if (rc) return rc;
     for (int k = 0; k < F_NUM; ++k) {
@@ -1726,7 +1807,6 @@ class DeviceImpl final : public Device {
   Derived<R> derived() {
     Derived<R> D;
     D.dens = dens_.p; D.press = press_.p; D.dinv = dinv_.p;
-    for (int a = 0; a < 6; ++a) D.pd[a] = pd_[a].p;
     for (int a = 0; a < 3; ++a) D.f[a] = f_[a].p;
     for (int a = 0; a < 6; ++a) D.dcdt[a] = dcdt_[a].p;
     for (int a = 0; a < 9; ++a) D.gv[a] = gv_[a].p;
@@ -1794,7 +1874,9 @@ class DeviceImpl final : public Device {
   DevBuf<double> x_[2], y_[2], z_[2];
   DevBuf<R> v_[2][3], c_[2][6], mass_[2];
   DevBuf<int> type_[2], id_[2];
-  DevBuf<R> dens_, press_, dinv_, pd_[6], f_[3], dcdt_[6], gv_[9];
+  DevBuf<R> dens_, press_, dinv_, f_[3], dcdt_[6], gv_[9];
+  DevBuf<PairRec<R>> rec_;
+  bool rec_valid_ = false;
   DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, flag_;
   DevBuf<int4> nl_idx_;
   DevBuf<int> nl_count_, nl_queue_, nl_qlen_;
