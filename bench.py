@@ -31,12 +31,13 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 
 def kernel_bytes(phase, dim, rbytes, store_gradv=False):
     """Algorithmic (compulsory) HBM bytes per particle for one launch of the
-    phase's main kernel: every array the kernel must read or write, once."""
+    phase's main kernel: every array the operation must read or write, once
+    (neighbour data re-reads and the neighbour list are not counted)."""
     pos = 8 * dim
     v, c = rbytes * dim, rbytes * (6 if dim == 3 else 3)
-    if phase == "forces":      # x v c dens press type mass -> F
-        return pos + v + c + 2 * rbytes + 4 + rbytes + v
-    if phase == "density":     # x mass -> dens press
+    if phase == "forces":      # k_pairs: x v c type dens press mass -> F dc/dt
+        return pos + v + c + 4 + 3 * rbytes + v + c
+    if phase == "density":     # k_density: x mass -> dens press
         return pos + rbytes + 2 * rbytes
     if phase == "gradv_dcdt":  # x v type dens c -> dcdt (+ gradv)
         return pos + v + 4 + rbytes + c + c + (rbytes * dim * dim if store_gradv else 0)
@@ -48,6 +49,23 @@ def kernel_bytes(phase, dim, rbytes, store_gradv=False):
         moving = pos + v + c + rbytes + 4 + 4
         return (pos + 8) + 20 + (8 + 4 + moving + moving + 8)
     return 0
+
+
+TRAFFIC_FILE = "profiles/traffic_walls_colloids_1M_fp32.json"
+
+
+def measured_traffic(kernel, N):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (scaled by a copy kernel of known size) on this workload, from
+    the committed profile; None when no measurement of this kernel and size."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if kernel not in t or f"N={N}" not in t.get("workload", ""):
+        return None
+    return t[kernel]["traffic_bytes"]
 
 
 def make_system(precision, n, n_colloids, tmpdir, seed=3):
@@ -137,6 +155,9 @@ def main():
     avg_ms = ph[dom][0] / ph[dom][1]
     bpp = kernel_bytes(dom, 3, 4)
     achieved = bpp * N / (avg_ms * 1e-3) / 1e9
+    kname = {"forces": "k_pairs", "density": "k_density", "neighbours": "k_rank_gather",
+             "vv1": "k_vv1", "vv2": "k_vv2", "walls_colloids": "k_walls"}.get(dom, dom)
+    traffic = measured_traffic(kname, N)
     step_ms = 1e3 * r32["elapsed"] / a.steps
     line = {
         "metric": "particle-steps/s, 3D 1M-particle Oldroyd-B (FP32 vs FP64)",
@@ -166,15 +187,14 @@ def main():
         },
         "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in ph.items() if v[1]},
         "roofline": {
-            "kernel": {"forces": "k_forces", "density": "k_density",
-                       "gradv_dcdt": "k_gradv", "neighbours": "neighbour search",
-                       "vv1": "k_vv1", "vv2": "k_vv2"}.get(dom, dom),
+            "kernel": kname,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": TRAFFIC_FILE if traffic is not None else None,
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,
         },

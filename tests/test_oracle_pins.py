@@ -8,7 +8,14 @@ wrong sign, a transposed index, a wrong constant) fails it:
   velocity gradient    exact for a linear field: gradv = M G^T with the lattice
                        moment matrix M computed independently
   forces               total momentum conservation; zero on a lattice at rest
-                       with c = I; pressure-only force vs an independent numpy sum
+                       with c = I; full pair force vs an independent numpy sum
+  external force       closed forms of the three force types
+  Morris boundaries    walls: exact for a linear shear flow (gradient and full
+                       force, with the fluid-partner density/pressure rule);
+                       walls and colloids: the momentum balance of mirrored
+                       pairs in closed form; colloids: zero gradient under a
+                       common translation, spin torque opposing and linear in
+                       the spin
   dc/dt                steady simple shear of Oldroyd-B: c_xy = tau*g, c_xx = 1 + 2(tau*g)^2
   VV                   hand-computed single-particle updates, wall velocities
   colloids             two-body repulsion and magnetic forces in closed form;
@@ -244,6 +251,238 @@ def test_forces_match_independent_numpy_sum(tmp_path):
         np.testing.assert_allclose(o.get(n), F[:, a], rtol=1e-9, atol=1e-10 * np.abs(F).max())
 
 
+# ---------------------------------------------------------------- external force
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_external_force_closed_form(kind, tmp_path):
+    """Fluid only: type 1 m f; type 2 m f_x sin(2 pi y / L1) along x; type 3
+    +m f_x below mid-channel and -m f_x above it."""
+    f = [0.7, -0.4, 0.25]
+    s = _sys(helpers.cfg_colloids_3d(n=20, ext_force_type=kind, ext_force=f), tmp_path)
+    helpers.perturb(s, pos=0.3)
+    p = s.params
+    o = oracle.Oracle.from_system(s)
+    rng = np.random.default_rng(kind)
+    F0 = [rng.normal(size=s.N) for _ in range(3)]
+    for a, n in enumerate(("fx", "fy", "fz")):
+        o.set(n, F0[a])
+    o.calculate_external_forces()
+    m, y, fl = o.get("mass"), o.get("y"), o.get("type") == 0
+    want = np.zeros((3, s.N))
+    if kind == 1:
+        want[:, fl] = np.outer(f, m[fl])
+    elif kind == 2:
+        want[0, fl] = m[fl] * f[0] * np.sin(2 * np.pi / p.L[1] * y[fl])
+    else:
+        want[0, fl] = np.where(y[fl] < 0.5 * p.L[1], 1.0, -1.0) * m[fl] * f[0]
+    for a, n in enumerate(("fx", "fy", "fz")):
+        np.testing.assert_allclose(o.get(n) - F0[a], want[a], rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------- Morris boundaries
+def _pairs_within(x, L, periodic, rcut):
+    """All ordered pairs (i, j), i != j, closer than rcut under the minimum image
+    along the periodic axes: (i, j, r_ij = x_i - x_j, |r_ij|)."""
+    r = x[:, None, :] - x[None, :, :]
+    for a in range(x.shape[1]):
+        if periodic[a]:
+            r[..., a] -= L[a] * np.round(r[..., a] / L[a])
+    rr = np.linalg.norm(r, axis=2)
+    i, j = np.nonzero((rr < rcut) & (rr > 0))
+    return i, j, r[i, j], rr[i, j]
+
+
+def _gradW(p, r):
+    q = 3 * r / p.rcut
+    return p.c_gradw * ((3 - q) ** 4 - 6 * np.where(q < 2, (2 - q) ** 4, 0)
+                        + 15 * np.where(q < 1, (1 - q) ** 4, 0))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wall_morris_is_exact_for_linear_shear(dim, tmp_path):
+    """Wall particles squeezed to within 0.2 dx of their wall while the nearest
+    fluid sits 0.5 dx from it keep beta = 1 + d_wall/d_fluid below its cap, so
+    the Morris velocity beta (v_i - V_wall) equals v(y_i) - v(y_j) for a flow
+    linear in y that takes the wall's velocity at the wall.  The oracle's
+    velocity gradient and its complete pair force on every fluid particle must
+    then equal plain SPH sums with those field differences, where a wall partner
+    j enters the force with the fluid particle's density and pressure and its own
+    conformation tensor (src/kernel_forces.cu:213-233)."""
+    if dim == 2:
+        cfg = helpers.cfg_couette(n=24, Vwall_bottom=-0.5, Vwall_top=0.3)
+    else:
+        cfg = helpers.cfg_colloids_3d(n=12, n_colloids=0, Vwall_bottom=-0.5, Vwall_top=0.3)
+        cfg.update(N_colloids=0, coll_x=[])
+    s = _sys(cfg, tmp_path)
+    p = s.params
+    rng = np.random.default_rng(5)
+    dx = p.L[0] / p.Nxyz[0]
+    t = s.get("type")
+    fl = t == 0
+    x = np.stack([s.get(n) for n in ("x", "y", "z")[:dim]], 1)
+    for a in [0] + ([2] if dim == 3 else []):          # jitter along the periodic axes
+        x[fl, a] = np.mod(x[fl, a] + rng.uniform(-0.3, 0.3, fl.sum()) * dx, p.L[a])
+    nb, nt = (t == 1).sum(), (t == 2).sum()
+    x[t == 1, 1] = p.y_bottom - rng.uniform(0.01, 0.2, nb) * dx
+    x[t == 2, 1] = p.y_top + rng.uniform(0.01, 0.2, nt) * dx
+    # Two linear fields: one for the lower half (bottom wall), one for the upper.
+    gb, gt = np.array([1.3, 0.4, -0.6])[:dim], np.array([-0.8, -0.5, 0.9])[:dim]
+    e0 = np.eye(dim)[0]
+    low = x[:, 1] < 0.5 * p.L[1]
+    field = np.where(low[:, None], p.V_bottom * e0 + (x[:, 1:2] - p.y_bottom) * gb,
+                     p.V_top * e0 + (x[:, 1:2] - p.y_top) * gt)
+    for a, n in enumerate(("x", "y", "z")[:dim]):
+        s.set(n, x[:, a])
+    for a, n in enumerate(("vx", "vy", "vz")[:dim]):
+        v = np.where(fl, field[:, a], 0.0)
+        s.set(n, v)
+    for n in comps("c", dim):                           # every particle, walls included
+        s.set(n, s.get(n) + rng.uniform(-0.3, 0.3, s.N))
+    o = oracle.Oracle.from_system(s)
+    o.neighbours_search()
+    o.calculate_density()
+    o.calculate_pressure()
+    o.calculate_vel_gradient()
+    o.calculate_pair_forces()
+    d, pr = o.get("dens"), o.get("press")
+    i, j, r, rr = _pairs_within(x, p.L[:dim], [True, False, True][:dim], p.rcut)
+    keep = fl[i]                                  # fluid i with any partner
+    i, j, r, rr = i[keep], j[keep], r[keep], rr[keep]
+    both_wall = ~fl[j]
+    assert both_wall.any()
+    beta = np.where(both_wall, 1 + np.abs(x[j, 1] - np.where(t[j] == 1, p.y_bottom, p.y_top))
+                    / np.abs(x[i, 1] - np.where(t[j] == 1, p.y_bottom, p.y_top)), 0)
+    assert beta.max() < p.beta_max
+    vij = field[i] - field[j]                    # the linear field at both positions
+    gw = _gradW(p, rr)
+    e = r / rr[:, None]
+    gv = np.zeros((s.N, dim, dim))
+    np.add.at(gv, i, -(gw / d[i])[:, None, None] * e[:, :, None] * vij[:, None, :])
+    got = np.stack([o.get(n) for n in comps("gradv", dim)], 1).reshape(-1, dim, dim)
+    scale = np.abs(gv[fl]).max()
+    np.testing.assert_allclose(got[fl], gv[fl], rtol=0, atol=1e-11 * scale)
+    # Pair force with the fluid-partner rule for wall j.
+    C = np.zeros((s.N, dim, dim))
+    idx = [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]
+    for (a, b), n in zip(idx, comps("c", dim)):
+        C[:, a, b] = C[:, b, a] = o.get(n)
+    dj = np.where(fl[j], d[j], d[i])
+    pj = np.where(fl[j], pr[j], pr[i])
+    Pi_i = pr[i][:, None, None] * np.eye(dim) - p.nkT * C[i]
+    Pi_j = pj[:, None, None] * np.eye(dim) - p.nkT * C[j]
+    P = Pi_i / (d[i] ** 2)[:, None, None] + Pi_j / (dj ** 2)[:, None, None]
+    pair = -gw[:, None] * np.einsum("kab,kb->ka", P, e)
+    pair += (gw / (d[i] * dj * rr))[:, None] * (p.a * vij + p.b * (e * vij).sum(1)[:, None] * e)
+    F = np.zeros((s.N, dim))
+    np.add.at(F, i, pair)
+    got = np.stack([o.get(n) for n in comps("f", dim)], 1)
+    np.testing.assert_allclose(got[fl], F[fl], rtol=0, atol=1e-10 * np.abs(F[fl]).max())
+
+
+@pytest.mark.parametrize("case", ["couette", "colloids_2d", "colloids_3d"])
+def test_momentum_balance_with_boundaries(case, tmp_path):
+    """A pair of a fluid particle f and a wall or colloid particle w is evaluated
+    from both sides with mirrored Morris velocities and f's density and pressure.
+    The stress terms differ: from f's side w enters with Pi_w = p_f - nkT c_w
+    (src/kernel_forces.cu:213-233), from w's side both particles take f's full
+    Pi_f (src/kernel_forces.cu:319-335).  So the pair forces over all particles
+    sum to  sum_(f,w) -gradW nkT (c_f - c_w) e_fw / d_f^2  exactly, which is zero
+    when the boundary particles carry the fluid's c; checked with moving walls
+    and translating, spinning colloids."""
+    cfg = {"couette": lambda: helpers.cfg_couette(n=32),
+           "colloids_2d": lambda: helpers.cfg_colloids_2d(n=40),
+           "colloids_3d": lambda: helpers.cfg_colloids_3d(n=20)}[case]()
+    s = _sys(cfg, tmp_path, noise=0.05)
+    helpers.perturb(s, pos=0.3, conf=0.0)
+    p = s.params
+    dim = p.dim
+    rng = np.random.default_rng(2)
+    o = oracle.Oracle.from_system(s)
+    if p.N_colloids:
+        for n in ("coll_vx", "coll_vy", "coll_vz", "coll_omegax", "coll_omegay", "coll_omegaz"):
+            o.set(n, rng.uniform(-0.5, 0.5, p.N_colloids))
+
+    def total_force():
+        o.neighbours_search()
+        o.calculate_density()
+        o.calculate_pressure()
+        o.calculate_pair_forces()
+        return np.stack([o.get(n) for n in comps("f", dim)], 1)
+
+    for n in comps("c", dim):       # c = I everywhere (colloid shells start at c = 0)
+        o.set(n, np.full(s.N, 1.0 if n in ("cxx", "cyy", "czz") else 0.0))
+    F = total_force()
+    t = o.get("type")
+    assert np.abs(F[t != 0]).max() > 0
+    np.testing.assert_allclose(F.sum(0), 0.0, atol=1e-13 * np.abs(F).sum())
+    C = np.zeros((s.N, dim, dim))
+    idx = [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]
+    for (a, b), n in zip(idx, comps("c", dim)):
+        c = o.get(n) + rng.uniform(-0.2, 0.2, s.N)      # fluid, walls and shells
+        o.set(n, c)
+        C[:, a, b] = C[:, b, a] = c
+    F = total_force()
+    x = np.stack([o.get(n) for n in comps("x", dim)], 1)
+    i, j, r, rr = _pairs_within(x, p.L[:dim], [True, not p.wall, True][:dim], p.rcut)
+    mixed = (t[i] == 0) & (t[j] != 0)
+    i, j, r, rr = i[mixed], j[mixed], r[mixed], rr[mixed]
+    e = r / rr[:, None]
+    d = o.get("dens")
+    coef = -_gradW(p, rr) * p.nkT / d[i] ** 2
+    want = (coef[:, None] * np.einsum("kab,kb->ka", C[i] - C[j], e)).sum(0)
+    assert np.abs(want).max() > 1e-6 * np.abs(F).sum() / s.N
+    np.testing.assert_allclose(F.sum(0), want, rtol=1e-9, atol=1e-13 * np.abs(F).sum())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_colloid_morris_galilean_and_spin(dim, tmp_path):
+    """(1) Fluid, walls and a non-spinning colloid all moving with one velocity
+    have no relative velocity: the velocity gradient of every fluid particle is
+    zero.  (2) With fluid and walls at rest, spinning colloid 0 adds a torque
+    that opposes the spin and is linear in it (the pressure part does not
+    depend on velocity and drops out of the difference)."""
+    V = 0.37
+    if dim == 2:
+        cfg = helpers.cfg_colloids_2d(n=40, Vwall_bottom=V, Vwall_top=V)
+    else:
+        cfg = helpers.cfg_colloids_3d(n=20, Vwall_bottom=V, Vwall_top=V)
+    s = _sys(cfg, tmp_path)
+    helpers.perturb(s, pos=0.3)
+    p = s.params
+    nc = p.N_colloids
+    s.set("vx", np.where(s.get("type") == 0, V, 0.0))
+    o = oracle.Oracle.from_system(s)
+    o.set("coll_vx", np.full(nc, V))
+    o.neighbours_search()
+    o.calculate_density()
+    o.calculate_vel_gradient()
+    fl = o.get("type") == 0
+    gscale = abs(p.c_gradw) * V / o.get("dens").min() * 100
+    for n in comps("gradv", dim):
+        assert np.abs(o.get(n)[fl]).max() <= 1e-13 * gscale, n
+
+    def torque(omega):
+        q = oracle.Oracle.from_system(s)
+        q.set("vx", np.zeros(s.N))
+        q.set("coll_vx", np.zeros(nc))
+        w = np.zeros((3, nc))
+        w[:, 0] = omega
+        for a, n in enumerate(("coll_omegax", "coll_omegay", "coll_omegaz")):
+            q.set(n, w[a])
+        q.neighbours_search()
+        q.calculate_density()
+        q.calculate_pressure()
+        q.calculate_pair_forces()
+        q.set_step(0)
+        q.calculate_force_colloids()
+        return np.array([q.get(n)[0] for n in ("tx_colloid", "ty_colloid", "tz_colloid")])
+
+    w = np.array([0.0, 0.0, 1.5]) if dim == 2 else np.array([0.6, -0.9, 1.2])
+    T0, T1, T2 = torque(0 * w), torque(w), torque(2 * w)
+    dT = T1 - T0
+    assert dT @ w < 0 and np.linalg.norm(dT) > 0
+    np.testing.assert_allclose(T2 - T0, 2 * dT, rtol=1e-9, atol=1e-12 * np.abs(dT).max())
+
+
 # ---------------------------------------------------------------- dc/dt
 def test_dcdt_steady_shear(tmp_path):
     s = _sys(helpers.cfg_sine(n=10), tmp_path)
@@ -423,6 +662,60 @@ def test_colloid_rotation_2d_is_exact(tmp_path):
     np.testing.assert_allclose(np.angle(np.exp(1j * (ang1 - ang0))), om * p.dt, atol=1e-12)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_colloid_rigid_body_step(dim, tmp_path):
+    """M = pi R^2 rho_c (2D) or 4/3 pi R^3 rho_c, I = M R^2/2 or 2/5 M R^2.
+    VV1: V += dt/(2M) F, omega += dt/(2I) T, X += V dt, and each shell particle
+    is placed at X + (offset rotated by |omega| dt about omega) (Rodrigues'
+    formula; the 3D quaternion path).  VV2: V += dt/(2M) F, omega += dt/(2I) T."""
+    s, o = _two_colloids(tmp_path, dim, 0.45, coll_rho=0.8)
+    p = s.params
+    R = p.coll_R
+    M = (math.pi * R * R if dim == 2 else 4.0 / 3.0 * math.pi * R ** 3) * 0.8
+    assert p.coll_mass == pytest.approx(M, rel=1e-15)
+    assert p.coll_I == pytest.approx((0.5 if dim == 2 else 0.4) * M * R * R, rel=1e-15)
+    rng = np.random.default_rng(9)
+    F = rng.normal(size=(3, 2)) * 1e-2
+    T = rng.normal(size=(3, 2)) * 1e-3
+    V = rng.normal(size=(3, 2)) * 0.3
+    W = rng.normal(size=(3, 2)) * 2.0
+    if dim == 2:
+        F[2] = T[:2] = V[2] = W[:2] = 0.0
+    for a, ax in enumerate("xyz"):
+        o.set(f"f{ax}_colloid", F[a])
+        o.set(f"t{ax}_colloid", T[a])
+        o.set(f"coll_v{ax}", V[a])
+        o.set(f"coll_omega{ax}", W[a])
+    t = o.get("type")
+    X0 = np.stack([o.get(f"coll_{ax}") for ax in "xyz"])
+    pos0 = np.stack([o.get(ax) for ax in "xyz"])
+    o.V_Verlet_step1()
+    V1 = V + 0.5 * p.dt / M * F
+    W1 = W + 0.5 * p.dt / p.coll_I * T
+    for a, ax in enumerate("xyz"[:dim]):
+        np.testing.assert_allclose(o.get(f"coll_v{ax}"), V1[a], rtol=1e-14, atol=1e-16)
+        np.testing.assert_allclose(o.get(f"coll_{ax}"), X0[a] + V1[a] * p.dt, rtol=1e-14)
+    for a, ax in enumerate("xyz"):
+        if dim == 3 or ax == "z":
+            np.testing.assert_allclose(o.get(f"coll_omega{ax}"), W1[a], rtol=1e-14, atol=1e-16)
+    X1 = np.stack([o.get(f"coll_{ax}") for ax in "xyz"])
+    pos1 = np.stack([o.get(ax) for ax in "xyz"])
+    for k in range(2):
+        sh = t == 3 + k
+        off0 = pos0[:, sh] - X0[:, k:k + 1]
+        w = W1[:, k]
+        ang = np.linalg.norm(w) * p.dt
+        u = w / np.linalg.norm(w)
+        want = (off0 * math.cos(ang) + np.cross(u, off0.T).T * math.sin(ang)
+                + np.outer(u, u @ off0) * (1 - math.cos(ang)))
+        got = pos1[:, sh] - X1[:, k:k + 1]
+        np.testing.assert_allclose(got[:dim], want[:dim], rtol=0, atol=1e-13)
+    o.V_Verlet_step2()
+    for a, ax in enumerate("xyz"[:dim]):
+        np.testing.assert_allclose(o.get(f"coll_v{ax}"), V1[a] + 0.5 * p.dt / M * F[a],
+                                   rtol=1e-14, atol=1e-16)
+
+
 def test_wall_force_is_sum_over_wall_particles(tmp_path):
     s = _sys(helpers.cfg_couette(n=32), tmp_path, noise=1e-2)
     helpers.perturb(s)
@@ -430,8 +723,6 @@ def test_wall_force_is_sum_over_wall_particles(tmp_path):
     o.neighbours_search()
     o.calculate_forces()
     t = o.get("type")
-    for w, n in ((1, "fx"), (2, "fy")):
-        pass
     fxw, fyw = o.get("fx_wall"), o.get("fy_wall")
     for w in (0, 1):
         sel = t == w + 1
