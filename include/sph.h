@@ -199,6 +199,50 @@ int sph_write_restart(sph_handle h, const char *path);
 int sph_set_timing(sph_handle h, int enable);
 int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]);
 
+/* ---- batches of independent simulations --------------------------------- */
+/* A batch runs many independent simulations of uneven size - a parameter
+ * sweep - in one device state: their arrays are concatenated and each phase of
+ * the step is one launch over all of them, captured once as a hipGraph and
+ * replayed every step.  The reference runs one simulation per process (each
+ * one is the loop of src/main.cu:358-459); there is no counterpart API.
+ * Every simulation of a batch gives, step by step, the same numbers as its own
+ * sph_step run (the same kernels on the same data).  All simulations must share
+ * dim and wall; they share the precision of the batch. */
+typedef struct sph_batch *sph_batch_handle;
+sph_batch_handle sph_batch_create(int precision);
+void sph_batch_destroy(sph_batch_handle b);
+const char *sph_batch_last_error(sph_batch_handle b);
+/* Copies the current state of an initialized system into the batch (before
+ * sph_batch_prepare); *index receives its position.  The system handle stays
+ * independent of the batch. */
+int sph_batch_add(sph_batch_handle b, sph_handle sim, int *index);
+int sph_batch_size(sph_batch_handle b);
+/* Uploads all simulations, then neighbours_search + calculate_forces at step 0
+ * (as sph_prepare). */
+int sph_batch_prepare(sph_batch_handle b);
+/* `nsteps` steps of every simulation (as sph_step).  Asynchronous. */
+int sph_batch_step(sph_batch_handle b, int nsteps);
+int sph_batch_synchronize(sph_batch_handle b);
+int sph_batch_current_step(sph_batch_handle b);
+/* Field of simulation `sim`, ID order, names and counts as sph_get_field. */
+int sph_batch_get_field(sph_batch_handle b, int sim, const char *name, void *dst,
+                        int64_t count);
+/* ke[s] for every simulation s (as sph_kinetic_energy). */
+int sph_batch_kinetic_energy(sph_batch_handle b, double *ke);
+/* out[8 s + k]: the sph_observables of simulation s. */
+int sph_batch_observables(sph_batch_handle b, double *out);
+/* Launch shape of the per-particle kernels: `block` threads per workgroup
+ * (64, 128 or 256) and `xcd` = 1 to map consecutive workgroups XCD by XCD.
+ * Results do not depend on it. */
+int sph_batch_set_launch(sph_batch_handle b, int block, int xcd);
+int sph_batch_get_launch(sph_batch_handle b, int *block, int *xcd);
+/* Times `steps` steps under each launch shape and keeps the fastest.  The
+ * timed steps are real steps of the batch (its step counter advances by
+ * 6 * (steps + 2)); since results do not depend on the shape, the trajectory
+ * is the one an untuned run would follow. */
+int sph_batch_autotune(sph_batch_handle b, int steps, int *block, int *xcd,
+                       double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,18 +4,18 @@
 // (src/class_system.h:146-651) and the driver loop (src/main.cu:30-561).  Host
 // set-up runs on the CPU; the first device call allocates and uploads the
 // state.  Methods that mirror a reference method synchronise before returning.
+// The sph_batch_* calls run many such systems in one device state.
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/sph.h"
 #include "sph_device.h"
 #include "sph_host.h"
-
-namespace sph {
-int device_forces_all(Device *d, int precision, bool store_gradv, std::string &err);
-}
 
 struct sph_system {
   sph::HostState hs;
@@ -24,6 +24,15 @@ struct sph_system {
   std::string err;
   int step = 0;
   bool timing = false;
+};
+
+struct sph_batch {
+  int precision = SPH_FP64;
+  std::vector<sph::HostState> sims;  // host copies, ID order
+  std::unique_ptr<sph::Device> dev;
+  std::string err;
+  int step = 0;
+  int block = 256, xcd = 1;
 };
 
 namespace {
@@ -37,18 +46,36 @@ int ensure_device(sph_handle h) {
     return SPH_ERR_STATE;
   }
   auto dev = sph::make_device(h->precision);
-  int rc = dev->upload(h->hs, h->err);
+  int rc = dev->upload({&h->hs}, h->err);
   if (rc) return rc;
   dev->set_timing(h->timing);
   h->dev = std::move(dev);
   return SPH_OK;
 }
 
-void magnetic_moment(const sph_params_t &p, int step, double m[3]) {
-  double ang = p.omega_magnet * (p.t0 + p.dt * (double)step);
-  m[0] = std::cos(ang);
-  m[1] = std::sin(ang);
-  m[2] = 0.0;
+// Steps run as captured hipGraphs unless SPH_NO_GRAPH is set.
+bool use_graph() {
+  static const bool on = std::getenv("SPH_NO_GRAPH") == nullptr;
+  return on;
+}
+
+// Rheology observables of one simulation from its device sums (sph.h).
+void finish_observables(const sph_params_t &p, int step, double ke, const double s[6],
+                        const double fwall_x[2], double out[8]) {
+  const double nan = std::nan("");
+  out[0] = ke;
+  out[1] = s[2] > 0 ? p.nkT * s[0] / s[2] : nan;
+  out[2] = s[2] > 0 ? p.nkT * s[1] / s[2] : nan;
+  out[3] = out[4] = nan;
+  if (p.wall) {
+    const double area = p.dim == 3 ? p.L[0] * p.L[2] : p.L[0];
+    out[3] = fwall_x[0] / area;
+    out[4] = fwall_x[1] / area;
+  }
+  auto half_turn = [](double a) { return a - M_PI * std::ceil(a / M_PI - 0.5); };
+  out[5] = s[5] > 0 ? half_turn(0.5 * std::atan2(s[3], s[4])) : nan;
+  out[6] = half_turn(p.omega_magnet * (p.t0 + p.dt * (double)step));
+  out[7] = half_turn(out[6] - out[5]);
 }
 
 struct FieldRef {
@@ -96,7 +123,7 @@ int sync_dev(sph_handle h) { return h->dev->synchronize(h->err); }
 int pull(sph_handle h, std::initializer_list<const char *> names) {
   if (!h->dev) return SPH_OK;
   for (const char *n : names) {
-    int rc = h->dev->download_field(h->hs, n, h->err);
+    int rc = h->dev->download_field(0, h->hs, n, h->err);
     if (rc) return rc;
   }
   return SPH_OK;
@@ -153,7 +180,7 @@ int sph_get_field(sph_handle h, const char *name, void *dst, int64_t count) {
     return SPH_ERR_ARG;
   }
   if (h->dev && f.device) {
-    int rc = h->dev->download_field(h->hs, name, h->err);
+    int rc = h->dev->download_field(0, h->hs, name, h->err);
     if (rc) return rc;
     find_field(h->hs, name, f);  // storage may have been reallocated
   }
@@ -177,20 +204,20 @@ int sph_set_field(sph_handle h, const char *name, const void *src, int64_t count
   }
   if (h->dev && f.device) {
     // Refresh the whole host copy first so untouched entries stay current.
-    int rc = h->dev->download_field(h->hs, name, h->err);
+    int rc = h->dev->download_field(0, h->hs, name, h->err);
     if (rc) return rc;
     find_field(h->hs, name, f);
   }
   if (f.d) std::memcpy(f.d, src, count * sizeof(double));
   else std::memcpy(f.i, src, count * sizeof(int32_t));
-  if (h->dev && f.device) return h->dev->upload_field(h->hs, name, h->err);
+  if (h->dev && f.device) return h->dev->upload_field(0, h->hs, name, h->err);
   return SPH_OK;
 }
 
 int sph_get_cell_list(sph_handle h, int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce) {
   if (!h) return SPH_ERR_ARG;
   if (!h->dev) { h->err = "no neighbours_search has run"; return SPH_ERR_STATE; }
-  return h->dev->cell_list(pc, pi, cs, ce, h->err);
+  return h->dev->cell_list(0, pc, pi, cs, ce, h->err);
 }
 
 #define DEVICE_CALL(expr)                          \
@@ -212,9 +239,7 @@ int sph_calculate_external_forces(sph_handle h) {
 }
 int sph_calculate_force_walls(sph_handle h) { DEVICE_CALL(h->dev->force_walls(h->err)); }
 int sph_calculate_force_colloids(sph_handle h) { DEVICE_CALL(h->dev->force_colloids(h->err)); }
-int sph_calculate_forces(sph_handle h) {
-  DEVICE_CALL(sph::device_forces_all(h->dev.get(), h->precision, true, h->err));
-}
+int sph_calculate_forces(sph_handle h) { DEVICE_CALL(h->dev->forces_all(true, h->err)); }
 int sph_V_Verlet_step1(sph_handle h) { DEVICE_CALL(h->dev->vv1(h->err)); }
 int sph_V_Verlet_step2(sph_handle h) { DEVICE_CALL(h->dev->vv2(h->err)); }
 
@@ -222,9 +247,7 @@ int sph_set_step(sph_handle h, int step) {
   if (!h) return SPH_ERR_ARG;
   int rc = ensure_device(h);
   if (rc) return rc;
-  double m[3];
-  magnetic_moment(h->hs.p, step, m);
-  h->dev->set_magnetic_moment(m);
+  if ((rc = h->dev->set_step(step, h->err))) return rc;
   h->step = step;
   return SPH_OK;
 }
@@ -234,7 +257,7 @@ int sph_prepare(sph_handle h) {
   int rc = sph_set_step(h, 0);
   if (rc) return rc;
   if ((rc = h->dev->neighbours_search(h->err))) return rc;
-  if ((rc = sph::device_forces_all(h->dev.get(), h->precision, true, h->err))) return rc;
+  if ((rc = h->dev->forces_all(true, h->err))) return rc;
   return sync_dev(h);
 }
 
@@ -242,17 +265,8 @@ int sph_step(sph_handle h, int nsteps) {
   if (!h || nsteps < 0) return SPH_ERR_ARG;
   int rc = ensure_device(h);
   if (rc) return rc;
-  for (int n = 0; n < nsteps; ++n) {
-    const int step = h->step + 1;
-    if ((rc = h->dev->vv1(h->err))) return rc;
-    if ((rc = h->dev->neighbours_search(h->err))) return rc;
-    double m[3];
-    magnetic_moment(h->hs.p, step, m);
-    h->dev->set_magnetic_moment(m);
-    if ((rc = sph::device_forces_all(h->dev.get(), h->precision, false, h->err))) return rc;
-    if ((rc = h->dev->vv2(h->err))) return rc;
-    h->step = step;
-  }
+  if ((rc = h->dev->run_steps(nsteps, h->step + 1, use_graph(), h->err))) return rc;
+  h->step += nsteps;
   return SPH_OK;
 }
 
@@ -275,24 +289,11 @@ int sph_observables(sph_handle h, double out[8]) {
   if (!h || !out) return SPH_ERR_ARG;
   int rc = ensure_device(h);
   if (rc) return rc;
-  if ((rc = h->dev->kinetic_energy(&out[0], h->err))) return rc;
-  double s[6];
+  double ke = 0.0, s[6];
+  if ((rc = h->dev->kinetic_energy(&ke, h->err))) return rc;
   if ((rc = h->dev->observable_sums(s, h->err))) return rc;
-  const sph_params_t &p = h->hs.p;
-  const double nan = std::nan("");
-  out[1] = s[2] > 0 ? p.nkT * s[0] / s[2] : nan;
-  out[2] = s[2] > 0 ? p.nkT * s[1] / s[2] : nan;
-  out[3] = out[4] = nan;
-  if (p.wall) {
-    if ((rc = pull(h, {"fx_wall"}))) return rc;
-    const double area = p.dim == 3 ? p.L[0] * p.L[2] : p.L[0];
-    out[3] = h->hs.fwall[0][0] / area;
-    out[4] = h->hs.fwall[0][1] / area;
-  }
-  auto half_turn = [](double a) { return a - M_PI * std::ceil(a / M_PI - 0.5); };
-  out[5] = s[5] > 0 ? half_turn(0.5 * std::atan2(s[3], s[4])) : nan;
-  out[6] = half_turn(p.omega_magnet * (p.t0 + p.dt * (double)h->step));
-  out[7] = half_turn(out[6] - out[5]);
+  if (h->hs.p.wall && (rc = pull(h, {"fx_wall"}))) return rc;
+  finish_observables(h->hs.p, h->step, ke, s, h->hs.fwall[0], out);
   return SPH_OK;
 }
 
@@ -369,6 +370,198 @@ int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]) {
     return SPH_OK;
   }
   return h->dev->get_timing(ms, calls, h->err);
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+sph_batch_handle sph_batch_create(int precision) {
+  if (precision != SPH_FP64 && precision != SPH_FP32) return nullptr;
+  sph_batch_handle b = new sph_batch();
+  b->precision = precision;
+  return b;
+}
+
+void sph_batch_destroy(sph_batch_handle b) { delete b; }
+
+const char *sph_batch_last_error(sph_batch_handle b) {
+  return b ? b->err.c_str() : "null handle";
+}
+
+int sph_batch_add(sph_batch_handle b, sph_handle h, int *index) {
+  if (!b || !h) return SPH_ERR_ARG;
+  if (b->dev) {
+    b->err = "simulations can only be added before sph_batch_prepare";
+    return SPH_ERR_STATE;
+  }
+  if (!h->hs.initialized) {
+    b->err = "the system is not initialized";
+    return SPH_ERR_STATE;
+  }
+  int rc;
+  if (h->dev && (rc = pull(h, {"x", "y", "z", "vx", "vy", "vz", "mass", "fx", "fy", "fz", "cxx",
+                              "cxy", "cyy", "cxz", "cyz", "czz", "type"}))) {
+    b->err = h->err;
+    return rc;
+  }
+  if (h->dev)
+    for (int k = 0; k < sph::C_NUM; ++k)
+      if ((rc = pull(h, {sph::kCFieldNames[k]}))) {
+        b->err = h->err;
+        return rc;
+      }
+  if (!b->sims.empty()) {
+    const sph_params_t &p0 = b->sims[0].p;
+    if (p0.dim != h->hs.p.dim || p0.wall != h->hs.p.wall) {
+      b->err = "all simulations of a batch must share dim and wall";
+      return SPH_ERR_ARG;
+    }
+  }
+  b->sims.push_back(h->hs);
+  if (index) *index = (int)b->sims.size() - 1;
+  return SPH_OK;
+}
+
+int sph_batch_size(sph_batch_handle b) { return b ? (int)b->sims.size() : -1; }
+
+int sph_batch_prepare(sph_batch_handle b) {
+  if (!b) return SPH_ERR_ARG;
+  if (b->sims.empty()) {
+    b->err = "the batch is empty";
+    return SPH_ERR_STATE;
+  }
+  std::vector<const sph::HostState *> v;
+  for (auto &s : b->sims) v.push_back(&s);
+  auto dev = sph::make_device(b->precision);
+  int rc;
+  if ((rc = dev->upload(v, b->err))) return rc;
+  if ((rc = dev->set_launch(b->block, b->xcd, b->err))) return rc;
+  b->dev = std::move(dev);
+  b->step = 0;
+  if ((rc = b->dev->set_step(0, b->err))) return rc;
+  if ((rc = b->dev->neighbours_search(b->err))) return rc;
+  if ((rc = b->dev->forces_all(true, b->err))) return rc;
+  return b->dev->synchronize(b->err);
+}
+
+#define BATCH_READY(b)                               \
+  do {                                               \
+    if (!(b)) return SPH_ERR_ARG;                    \
+    if (!(b)->dev) {                                 \
+      (b)->err = "sph_batch_prepare has not run";    \
+      return SPH_ERR_STATE;                          \
+    }                                                \
+  } while (0)
+
+int sph_batch_step(sph_batch_handle b, int nsteps) {
+  BATCH_READY(b);
+  if (nsteps < 0) return SPH_ERR_ARG;
+  int rc = b->dev->run_steps(nsteps, b->step + 1, use_graph(), b->err);
+  if (rc) return rc;
+  b->step += nsteps;
+  return SPH_OK;
+}
+
+int sph_batch_synchronize(sph_batch_handle b) {
+  BATCH_READY(b);
+  return b->dev->synchronize(b->err);
+}
+
+int sph_batch_current_step(sph_batch_handle b) { return b ? b->step : -1; }
+
+int sph_batch_get_field(sph_batch_handle b, int sim, const char *name, void *dst,
+                        int64_t count) {
+  if (!b || !name || (!dst && count)) return SPH_ERR_ARG;
+  if (sim < 0 || sim >= (int)b->sims.size()) {
+    b->err = "simulation index out of range";
+    return SPH_ERR_ARG;
+  }
+  sph::HostState &hs = b->sims[sim];
+  FieldRef f;
+  if (!find_field(hs, name, f)) {
+    b->err = std::string("unknown field ") + name;
+    return SPH_ERR_ARG;
+  }
+  if (f.n != count) {
+    b->err = std::string("field ") + name + " has " + std::to_string(f.n) + " entries";
+    return SPH_ERR_ARG;
+  }
+  if (b->dev && f.device) {
+    int rc = b->dev->download_field(sim, hs, name, b->err);
+    if (rc) return rc;
+    find_field(hs, name, f);
+  }
+  if (f.d) std::memcpy(dst, f.d, count * sizeof(double));
+  else std::memcpy(dst, f.i, count * sizeof(int32_t));
+  return SPH_OK;
+}
+
+int sph_batch_kinetic_energy(sph_batch_handle b, double *ke) {
+  BATCH_READY(b);
+  if (!ke) return SPH_ERR_ARG;
+  return b->dev->kinetic_energy(ke, b->err);
+}
+
+int sph_batch_observables(sph_batch_handle b, double *out) {
+  BATCH_READY(b);
+  if (!out) return SPH_ERR_ARG;
+  const int S = (int)b->sims.size();
+  std::vector<double> ke(S), sums(6 * S);
+  int rc;
+  if ((rc = b->dev->kinetic_energy(ke.data(), b->err))) return rc;
+  if ((rc = b->dev->observable_sums(sums.data(), b->err))) return rc;
+  for (int s = 0; s < S; ++s) {
+    sph::HostState &hs = b->sims[s];
+    if (hs.p.wall && (rc = b->dev->download_field(s, hs, "fx_wall", b->err))) return rc;
+    finish_observables(hs.p, b->step, ke[s], &sums[6 * s], hs.fwall[0], out + 8 * s);
+  }
+  return SPH_OK;
+}
+
+int sph_batch_set_launch(sph_batch_handle b, int block, int xcd) {
+  if (!b) return SPH_ERR_ARG;
+  if (block != 64 && block != 128 && block != 256) {
+    b->err = "block must be 64, 128 or 256 threads";
+    return SPH_ERR_ARG;
+  }
+  b->block = block;
+  b->xcd = xcd ? 1 : 0;
+  return b->dev ? b->dev->set_launch(block, b->xcd, b->err) : SPH_OK;
+}
+
+int sph_batch_get_launch(sph_batch_handle b, int *block, int *xcd) {
+  if (!b || !block || !xcd) return SPH_ERR_ARG;
+  *block = b->block;
+  *xcd = b->xcd;
+  return SPH_OK;
+}
+
+int sph_batch_autotune(sph_batch_handle b, int steps, int *block, int *xcd,
+                       double *ms_per_step) {
+  BATCH_READY(b);
+  if (steps < 1) return SPH_ERR_ARG;
+  static const int shapes[6][2] = {{256, 1}, {128, 1}, {64, 1}, {256, 0}, {128, 0}, {64, 0}};
+  double best = -1.0;
+  int bb = b->block, bx = b->xcd, rc;
+  for (auto &sh : shapes) {
+    if ((rc = sph_batch_set_launch(b, sh[0], sh[1]))) return rc;
+    // one untimed step captures the graphs of both buffer parities
+    if ((rc = sph_batch_step(b, 2)) || (rc = sph_batch_synchronize(b))) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = sph_batch_step(b, steps)) || (rc = sph_batch_synchronize(b))) return rc;
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count() / steps;
+    if (best < 0 || ms < best) {
+      best = ms;
+      bb = sh[0];
+      bx = sh[1];
+    }
+  }
+  if ((rc = sph_batch_set_launch(b, bb, bx))) return rc;
+  if (block) *block = bb;
+  if (xcd) *xcd = bx;
+  if (ms_per_step) *ms_per_step = best;
+  return SPH_OK;
 }
 
 }  // extern "C"

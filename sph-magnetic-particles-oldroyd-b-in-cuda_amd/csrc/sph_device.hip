@@ -1,4 +1,5 @@
-// sph_device.hip - gfx950 kernels and the device-side owner of one simulation.
+// sph_device.hip - gfx950 kernels and the device-side owner of a batch of
+// simulations (a single simulation is a batch of one).
 //
 // Layout: particles live in cell-sorted order as structure-of-arrays.  Positions
 // are double in both precision forms; every other per-particle field is `R`
@@ -24,6 +25,14 @@
 //   k_coll_vv1, k_coll_place, k_coll_vv2
 //                        kernel_move_colloids_VV_part1.cu:15-204, kernel_move_colloids_VV_part2.cu:14-47
 //   k_ke_partial, k_ke_final  kernel_calculate_macro_vars.cu:14-40 (deterministic FP64)
+//   k_set_step           magnetic moment of src/main.cu:290-293, 395-400
+//
+// Batches: the simulations' arrays are concatenated (particles, cells,
+// colloids, wall and colloid lists) and every launch covers the whole batch.
+// Particle IDs, cell indices and list entries are global (offset by the
+// simulation's first particle / cell), so the hash, scan and reorder need no
+// per-simulation logic and each simulation stays one contiguous range of the
+// sorted arrays.  A kernel finds its simulation's parameters through SimTable.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,10 +86,51 @@ struct Bodies {  // colloids (double) and wall/colloid lists (particle IDs)
   double *s[C_NUM];
   const int *clist, *cstart, *cowner;
   const double *xc, *yc, *zc;
-  const int *wlist, *wstart;
-  double *fwall;  // [3][2]
-  int n_colloids;
+  const int *wlist;
+  double *fwall;  // [nsim][3][2]
 };
+
+// Where each simulation of the batch sits.  Per-particle launches use blocks
+// that never straddle two simulations: tile b covers particles blk_first[b] ..
+// blk_first[b] + blockDim.x - 1 of simulation blk_sim[b] (the last tile of a
+// simulation is partial).
+struct SimTable {
+  const sph_params_t *P;  // [nsim]
+  const int *blk_sim;     // [nblk]
+  const int *blk_first;   // [nblk]
+  const int *pbeg;        // [nsim + 1] first particle of each simulation
+  const int *cell_off;    // [nsim] first cell
+  const int *coll_off;    // [nsim + 1] first colloid
+  const int *coll_sim;    // [total colloids] owning simulation
+  const int *wrange;      // [2 nsim + 1] wall-list ranges: wall w of sim s is [2s+w, 2s+w+1)
+  int nsim, nblk, ntotal, xcd;
+};
+
+// Tile of this workgroup: XCD-aware (see xcd_tile) or plain.
+__device__ __forceinline__ int xcd_tile();
+__device__ __forceinline__ int tile_of(const SimTable &T) {
+  return T.xcd ? xcd_tile() : (int)blockIdx.x;
+}
+
+// Simulation of global particle i (binary search over the particle offsets).
+__device__ __forceinline__ int sim_of(const SimTable &T, int i) {
+  int lo = 0, hi = T.nsim - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.pbeg[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The bodies of simulation s: colloid arrays shifted to its first colloid, so
+// a particle of type 3 + k addresses colloid k of its own simulation.
+__device__ __forceinline__ Bodies bodies_of(const Bodies &B, const SimTable &T, int s) {
+  Bodies r = B;
+  const int off = T.coll_off[s];
+#pragma unroll
+  for (int k = 0; k < C_NUM; ++k) r.s[k] = B.s[k] + off;
+  return r;
+}
 
 // ---------------------------------------------------------------------------
 // small device helpers
@@ -361,13 +411,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // ---------------------------------------------------------------------------
 // kernel_cell_list (src/kernel_cell_list.cu:15-63): clamp c == Ncells for x, z
 // and for y without walls; out-of-grid cells raise the device flag.
-__global__ __launch_bounds__(kBlock) void k_hash(sph_params_t P, const double *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int *cell, int *slot,
-                                                 int *count, int *flag, int N) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int c = -2;  // lanes past N
-  if (i < N) {
+                                                 int *count, int *flag) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const sph_params_t &P = T.P[sim];
+  const int i = T.blk_first[b] + threadIdx.x;
+  int c = -2;  // lanes past the simulation's last particle
+  if (i < T.pbeg[sim + 1]) {
     int cx = (int)floor(x[i] / P.cell_size[0]);
     if (cx == P.Ncells[0]) cx = cx - 1;
     int cy;
@@ -382,10 +434,10 @@ __global__ __launch_bounds__(kBlock) void k_hash(sph_params_t P, const double *_
       if (cz == P.Ncells[2]) cz = cz - 1;
       bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] && cz >= 0 &&
                 cz < P.Ncells[2];
-      c = ok ? cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
+      c = ok ? T.cell_off[sim] + cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
     } else {
       bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1];
-      c = ok ? cy * P.Ncells[0] + cx : -1;
+      c = ok ? T.cell_off[sim] + cy * P.Ncells[0] + cx : -1;
     }
     cell[i] = c;
     if (c == -1) atomicOr(flag, 1);
@@ -586,12 +638,16 @@ __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Part
 }
 
 template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                    PairRec<R> *rec, const int *__restrict__ cs,
-                                                    NList L, int N, int write_dens) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, Derived<R> D,
+                                                    PairRec<R> *rec, const int *__restrict__ csg,
+                                                    NList L, int write_dens) {
+  const int b = tile_of(T), sim = T.blk_sim[b];
+  const sph_params_t &P = T.P[sim];
+  const int *__restrict__ cs = csg + T.cell_off[sim];
+  const int N = T.ntotal;  // list stride
+  const int i = T.blk_first[b] + threadIdx.x;
   if (i == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
-  if (i >= N) return;
+  if (i >= T.pbeg[sim + 1]) return;
   const double *__restrict__ X = S.x;
   const double *__restrict__ Y = S.y;
   const double *__restrict__ Z = S.z;
@@ -633,11 +689,12 @@ __global__ __launch_bounds__(kBlock) void k_density(sph_params_t P, Particles<R>
 }
 
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_pressure(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                     PairRec<R> *rec, int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  pressure_terms<R>(P, S, D, rec, i, D.dens[i]);
+__global__ __launch_bounds__(kBlock) void k_pressure(SimTable T, Particles<R> S, Derived<R> D,
+                                                     PairRec<R> *rec) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1]) return;
+  pressure_terms<R>(T.P[sim], S, D, rec, i, D.dens[i]);
 }
 
 // Pi / d^2 with Pi = p I - nkT c, components xx xy yy xz yz zz
@@ -855,12 +912,15 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // DO_GV / DO_F select the parts.  A particle with more neighbours than the list
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                  Bodies B, const PairRec<R> *__restrict__ rec,
-                                                  NList L, int N, int store_gv, int write_dcdt,
-                                                  int with_ext) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+__global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
+    SimTable T, Particles<R> S, Derived<R> D, Bodies Ball, const PairRec<R> *__restrict__ rec,
+    NList L, int store_gv, int write_dcdt, int with_ext) {
+  const int b = tile_of(T), sim = T.blk_sim[b];
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1]) return;
+  const sph_params_t &P = T.P[sim];
+  const Bodies B = bodies_of(Ball, T, sim);
+  const int N = T.ntotal;  // list stride
   PairSelf<R> me;
   load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
   PairAcc<R> acc;
@@ -901,14 +961,18 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(sph_param
 // with the density pass's arithmetic (same hits, same order), skip the hits the
 // list held and add the rest, then finish as k_pairs would have.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__global__ __launch_bounds__(kBlock) void k_pairs_tail(sph_params_t P, Particles<R> S,
-                                                       Derived<R> D, Bodies B,
+__global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> S,
+                                                       Derived<R> D, Bodies Ball,
                                                        const PairRec<R> *__restrict__ rec, NList L,
-                                                       const int *__restrict__ cs, int store_gv,
+                                                       const int *__restrict__ csg, int store_gv,
                                                        int write_dcdt, int with_ext) {
   const int n = *L.qlen;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const int i = L.queue[q];
+    const int sim = sim_of(T, i);
+    const sph_params_t &P = T.P[sim];
+    const Bodies B = bodies_of(Ball, T, sim);
+    const int *__restrict__ cs = csg + T.cell_off[sim];
     PairSelf<R> me;
     load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
     PairAcc<R> acc;
@@ -940,10 +1004,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(sph_params_t P, Particles
 }
 
 template <class R, int DIM>
-__global__ __launch_bounds__(kBlock) void k_ext(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || S.type[i] != 0) return;
+__global__ __launch_bounds__(kBlock) void k_ext(SimTable T, Particles<R> S, Derived<R> D) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const sph_params_t &P = T.P[sim];
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1] || S.type[i] != 0) return;
   R f[3] = {D.f[0][i], D.f[1][i], DIM == 3 ? D.f[2][i] : R(0)};
   ext_force<R, DIM>(P, S.mass[i], S.y[i], f);
   D.f[0][i] = f[0];
@@ -955,11 +1020,12 @@ __global__ __launch_bounds__(kBlock) void k_ext(sph_params_t P, Particles<R> S, 
 // wall and colloid force collection (FP64, deterministic block reductions)
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(kRedBlock) void k_walls(sph_params_t P, Derived<R> D, Bodies B,
+__global__ __launch_bounds__(kRedBlock) void k_walls(SimTable T, Derived<R> D, Bodies B,
                                                      const int *__restrict__ rank) {
-  const int w = blockIdx.x;
-  const int j0 = B.wstart[w];
-  const int j1 = w < P.Nwalls - 1 ? B.wstart[w + 1] : P.Nlist_walls;
+  // block (w, s): wall w of simulation s; the list holds global particle IDs
+  const int w = blockIdx.x, sim = blockIdx.y;
+  const sph_params_t &P = T.P[sim];
+  const int j0 = T.wrange[2 * sim + w], j1 = T.wrange[2 * sim + w + 1];
   double acc[3] = {0.0, 0.0, 0.0};
   for (int j = j0 + threadIdx.x; j < j1; j += kRedBlock) {
     int p = rank[B.wlist[j]];
@@ -969,19 +1035,24 @@ __global__ __launch_bounds__(kRedBlock) void k_walls(sph_params_t P, Derived<R> 
   }
   block_sum<3, kRedBlock>(acc);
   if (threadIdx.x == 0) {
-    B.fwall[0 * 2 + w] = acc[0];
-    B.fwall[1 * 2 + w] = acc[1];
-    B.fwall[2 * 2 + w] = P.dim == 3 ? acc[2] : 0.0;
+    double *fw = B.fwall + 6 * sim;
+    fw[0 * 2 + w] = acc[0];
+    fw[1 * 2 + w] = acc[1];
+    fw[2 * 2 + w] = P.dim == 3 ? acc[2] : 0.0;
   }
 }
 
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_colloids(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                     Bodies B, const int *__restrict__ rank,
-                                                     double m0, double m1, double m2) {
-  const int ci = blockIdx.x;
+__global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S, Derived<R> D,
+                                                     Bodies Ball, const int *__restrict__ rank,
+                                                     const double *__restrict__ mag_all) {
+  // one block per colloid of the batch; ci is its index in its simulation
+  const int g = blockIdx.x, sim = T.coll_sim[g];
+  const sph_params_t &P = T.P[sim];
+  const Bodies B = bodies_of(Ball, T, sim);
+  const int ci = g - T.coll_off[sim], ncoll = P.N_colloids;
   const bool d3 = P.dim == 3;
-  const int j0 = B.cstart[ci], j1 = B.cstart[ci + 1];
+  const int j0 = Ball.cstart[g], j1 = Ball.cstart[g + 1];
   const double Xc = B.s[C_X][ci], Yc = B.s[C_Y][ci], Zc = B.s[C_Z][ci];
   double acc[6] = {0, 0, 0, 0, 0, 0};  // fx fy fz tx ty tz
   for (int j = j0 + threadIdx.x; j < j1; j += kBlock) {
@@ -1001,12 +1072,12 @@ __global__ __launch_bounds__(kBlock) void k_colloids(sph_params_t P, Particles<R
   block_sum<6, kBlock>(acc);
   if (threadIdx.x != 0) return;
   double F[3] = {acc[0], acc[1], acc[2]};
-  if (B.n_colloids >= 2) {
-    const double mag[3] = {m0, m1, m2};
+  if (ncoll >= 2) {
+    const double mag[3] = {mag_all[3 * sim], mag_all[3 * sim + 1], mag_all[3 * sim + 2]};
     // Repulsion, then magnetic force, over all other colloids within their
     // cut-offs (the set the reference finds through its colloid cell list).
     for (int pass = 0; pass < 2; ++pass) {
-      for (int j = 0; j < B.n_colloids; ++j) {
+      for (int j = 0; j < ncoll; ++j) {
         if (j == ci) continue;
         double r[3];
         r[0] = mimg(Xc - B.s[C_X][j], P.L[0], 0.5 * P.L[0]);
@@ -1048,10 +1119,11 @@ __global__ __launch_bounds__(kBlock) void k_colloids(sph_params_t P, Particles<R
 // integrators
 // ---------------------------------------------------------------------------
 template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_vv1(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+__global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Derived<R> D) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const sph_params_t &P = T.P[sim];
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1]) return;
   const int t = S.type[i];
   if (t > 2) return;
   double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
@@ -1087,10 +1159,11 @@ __global__ __launch_bounds__(kBlock) void k_vv1(sph_params_t P, Particles<R> S, 
 }
 
 template <class R, int DIM>
-__global__ __launch_bounds__(kBlock) void k_vv2(sph_params_t P, Particles<R> S, Derived<R> D,
-                                                int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || S.type[i] != 0) return;
+__global__ __launch_bounds__(kBlock) void k_vv2(SimTable T, Particles<R> S, Derived<R> D) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const sph_params_t &P = T.P[sim];
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1] || S.type[i] != 0) return;
   const R hdt = (R)(0.5 * P.dt);
   const R hdtm = hdt / S.mass[i];
   S.v[0][i] = S.v[0][i] + hdtm * D.f[0][i];
@@ -1108,11 +1181,14 @@ __device__ __forceinline__ void quat_mul(const double c[4], const double d[4], d
   r[3] = c[0] * d[3] + c[1] * d[2] - c[2] * d[1] + c[3] * d[0];
 }
 
-// Colloid half-kick, drift and rotation (one thread per colloid).  A 3D colloid
+// Colloid half-kick, drift and rotation (one thread per colloid of the batch;
+// simulations with coll_move != 0 keep their colloids fixed).  A 3D colloid
 // with |omega| == 0 keeps its orientation (the reference divides by zero).
-__global__ void k_coll_vv1(sph_params_t P, Bodies B) {
+__global__ void k_coll_vv1(SimTable T, Bodies B, int ncoll) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.n_colloids) return;
+  if (i >= ncoll) return;
+  const sph_params_t &P = T.P[T.coll_sim[i]];
+  if (P.coll_move != 0) return;
   double **C = B.s;
   const bool d3 = P.dim == 3;
   const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
@@ -1167,11 +1243,13 @@ __global__ void k_coll_vv1(sph_params_t P, Bodies B) {
 
 // Rigidly re-places every colloid boundary particle (one thread per list entry).
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_coll_place(sph_params_t P, Particles<R> S, Bodies B,
+__global__ __launch_bounds__(kBlock) void k_coll_place(SimTable T, Particles<R> S, Bodies B,
                                                        const int *__restrict__ rank, int nlist) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nlist) return;
-  const int ci = B.cowner[j];
+  const int ci = B.cowner[j];  // colloid of the batch
+  const sph_params_t &P = T.P[T.coll_sim[ci]];
+  if (P.coll_move != 0) return;
   double *const *C = B.s;
   double px, py, pz = 0.0;
   if (P.dim == 2) {
@@ -1206,9 +1284,11 @@ __global__ __launch_bounds__(kBlock) void k_coll_place(sph_params_t P, Particles
   if (P.dim == 3) S.z[p] = Z;
 }
 
-__global__ void k_coll_vv2(sph_params_t P, Bodies B) {
+__global__ void k_coll_vv2(SimTable T, Bodies B, int ncoll) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.n_colloids) return;
+  if (i >= ncoll) return;
+  const sph_params_t &P = T.P[T.coll_sim[i]];
+  if (P.coll_move != 0) return;
   double **C = B.s;
   const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
   C[C_VX][i] += hdtm * C[C_FX][i];
@@ -1224,23 +1304,44 @@ __global__ void k_coll_vv2(sph_params_t P, Bodies B) {
 // ---------------------------------------------------------------------------
 // kinetic energy: fixed partition + fixed reduction tree -> bitwise repeatable
 // ---------------------------------------------------------------------------
+// Block (b, s) sums particles b*kBlock + t + k*gridDim.x*kBlock of simulation s.
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_ke_partial(sph_params_t P, Particles<R> S,
-                                                       double *part, int N) {
+__global__ __launch_bounds__(kBlock) void k_ke_partial(SimTable T, Particles<R> S, double *part) {
+  const int sim = blockIdx.y;
+  const int p0 = T.pbeg[sim], n = T.pbeg[sim + 1] - p0;
+  const bool d3 = T.P[sim].dim == 3;
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
-    double vx = S.v[0][i], vy = S.v[1][i], vz = P.dim == 3 ? (double)S.v[2][i] : 0.0;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const int i = p0 + k;
+    double vx = S.v[0][i], vy = S.v[1][i], vz = d3 ? (double)S.v[2][i] : 0.0;
     acc[0] += 0.5 * (double)S.mass[i] * (vx * vx + vy * vy + vz * vz);
   }
   block_sum<1, kBlock>(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+  if (threadIdx.x == 0) part[sim * gridDim.x + blockIdx.x] = acc[0];
 }
 
+// Block s: the kinetic energy of simulation s from its n partial sums.
 __global__ __launch_bounds__(kBlock) void k_ke_final(const double *part, int n, double *out) {
+  const int sim = blockIdx.x;
   double acc[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += kBlock) acc[0] += part[i];
+  for (int i = threadIdx.x; i < n; i += kBlock) acc[0] += part[sim * n + i];
   block_sum<1, kBlock>(acc);
-  if (threadIdx.x == 0) out[0] = acc[0];
+  if (threadIdx.x == 0) out[sim] = acc[0];
+}
+
+// Step counter and magnetic moment m = (cos a, sin a, 0), a = omega (t0 + dt step)
+// of every simulation (src/main.cu:290-293, 395-400).  `step` >= 0 sets the
+// counter; -1 advances it by one (the form a captured graph replays).
+__global__ void k_set_step(SimTable T, int *steps, double *mag, int step) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= T.nsim) return;
+  const sph_params_t &P = T.P[s];
+  const int k = step >= 0 ? step : steps[s] + 1;
+  steps[s] = k;
+  const double ang = P.omega_magnet * (P.t0 + P.dt * (double)k);
+  mag[3 * s] = cos(ang);
+  mag[3 * s + 1] = sin(ang);
+  mag[3 * s + 2] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1248,9 +1349,13 @@ __global__ __launch_bounds__(kBlock) void k_ke_final(const double *part, int n, 
 // ---------------------------------------------------------------------------
 // Per block: sum over fluid particles of (cxx - cyy), cxy and the count.
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_polymer_partial(Particles<R> S, double *part, int N) {
+__global__ __launch_bounds__(kBlock) void k_polymer_partial(SimTable T, Particles<R> S,
+                                                            double *part) {
+  const int sim = blockIdx.y;
+  const int p0 = T.pbeg[sim], n = T.pbeg[sim + 1] - p0;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const int i = p0 + k;
     if (S.type[i] != 0) continue;
     acc[0] += (double)S.c[0][i] - (double)S.c[2][i];
     acc[1] += (double)S.c[1][i];
@@ -1258,20 +1363,23 @@ __global__ __launch_bounds__(kBlock) void k_polymer_partial(Particles<R> S, doub
   }
   block_sum<3, kBlock>(acc);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = acc[k];
+    for (int k = 0; k < 3; ++k) part[3 * (sim * gridDim.x + blockIdx.x) + k] = acc[k];
 }
 
-// One block: finishes the polymer sums and the colloid chain orientation,
-// the nematic mean of the in-plane angle of every colloid pair closer than the
-// magnetic cut-off.  out: [0] sum(cxx-cyy) [1] sum(cxy) [2] fluid count
-// [3] sum sin(2 phi) [4] sum cos(2 phi) [5] pair count.
-__global__ __launch_bounds__(kBlock) void k_observables_final(sph_params_t P, Bodies B,
+// Block s: finishes the polymer sums of simulation s and its colloid chain
+// orientation, the nematic mean of the in-plane angle of every colloid pair
+// closer than the magnetic cut-off.  out[6 s + k]: [0] sum(cxx-cyy) [1] sum(cxy)
+// [2] fluid count [3] sum sin(2 phi) [4] sum cos(2 phi) [5] pair count.
+__global__ __launch_bounds__(kBlock) void k_observables_final(SimTable T, Bodies Ball,
                                                               const double *part, int nparts,
                                                               double *out) {
+  const int sim = blockIdx.x;
+  const sph_params_t &P = T.P[sim];
+  const Bodies B = bodies_of(Ball, T, sim);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nparts; b += kBlock)
-    for (int k = 0; k < 3; ++k) acc[k] += part[3 * b + k];
-  const int nc = B.n_colloids;
+    for (int k = 0; k < 3; ++k) acc[k] += part[3 * (sim * nparts + b) + k];
+  const int nc = P.N_colloids;
   const long long npairs = (long long)nc * nc;
   for (long long q = threadIdx.x; q < npairs; q += kBlock) {
     const int i = (int)(q / nc), j = (int)(q % nc);
@@ -1288,7 +1396,7 @@ __global__ __launch_bounds__(kBlock) void k_observables_final(sph_params_t P, Bo
   }
   block_sum<6, kBlock>(acc);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 6; ++k) out[k] = acc[k];
+    for (int k = 0; k < 6; ++k) out[6 * sim + k] = acc[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1302,15 +1410,29 @@ struct DevBuf {
     if (p) (void)hipFree(p);
   }
   hipError_t alloc(size_t count) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
     n = count;
     return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
   }
+};
+
+// Where one simulation of the batch sits in the concatenated arrays.
+struct SimInfo {
+  sph_params_t p;
+  int pbeg, n;            // particles
+  int cell_off;           // cells
+  int coll_off;           // colloids
+  int clist_off, wlist_off;
 };
 
 template <class R>
 class DeviceImpl final : public Device {
  public:
   ~DeviceImpl() override {
+    drop_graphs();
     for (auto &e : events_) {
       (void)hipEventDestroy(e.a);
       (void)hipEventDestroy(e.b);
@@ -1318,11 +1440,42 @@ class DeviceImpl final : public Device {
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
-  int upload(const HostState &hs, std::string &err) override {
-    P_ = hs.p;
-    N_ = P_.N;
+  int nsim() const override { return (int)sims_.size(); }
+
+  int upload(const std::vector<const HostState *> &hs, std::string &err) override {
+    if (hs.empty()) {
+      err = "no simulation to upload";
+      return SPH_ERR_ARG;
+    }
+    P_ = hs[0]->p;
+    sims_.clear();
+    N_ = ncells_ = ncoll_ = nclist_ = nwlist_ = 0;
+    any_walls_ = false;
+    for (const HostState *h : hs) {
+      const sph_params_t &p = h->p;
+      if (p.dim != P_.dim || p.wall != P_.wall) {
+        err = "all simulations of a batch must share dim and wall";
+        return SPH_ERR_ARG;
+      }
+      SimInfo si;
+      si.p = p;
+      si.pbeg = N_;
+      si.n = p.N;
+      si.cell_off = ncells_;
+      si.coll_off = ncoll_;
+      si.clist_off = nclist_;
+      si.wlist_off = nwlist_;
+      sims_.push_back(si);
+      N_ += p.N;
+      ncells_ += p.Ntotal_cells;
+      ncoll_ += p.N_colloids;
+      nclist_ += p.Nlist_colloids;
+      nwlist_ += p.wall ? p.Nlist_walls : 0;
+      any_walls_ = any_walls_ || (p.wall && p.Nwalls > 0);
+    }
+    const int S = nsim();
     nc_ = P_.dim == 3 ? 6 : 3;
-    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (int b = 0; b < 2; ++b) {
       HIPCHK(x_[b].alloc(N_)); HIPCHK(y_[b].alloc(N_)); HIPCHK(z_[b].alloc(N_));
       for (int a = 0; a < 3; ++a) HIPCHK(v_[b][a].alloc(N_));
@@ -1335,21 +1488,24 @@ class DeviceImpl final : public Device {
     for (int a = 0; a < 9; ++a) HIPCHK(gv_[a].alloc(N_));
     HIPCHK(cell_.alloc(N_)); HIPCHK(slot_.alloc(N_)); HIPCHK(rank_.alloc(N_));
     HIPCHK(stage_src_.alloc(N_)); HIPCHK(stage_id_.alloc(N_)); HIPCHK(sorted_cell_.alloc(N_));
-    HIPCHK(count_.alloc(P_.Ntotal_cells)); HIPCHK(cs_.alloc(P_.Ntotal_cells + 1));
+    HIPCHK(count_.alloc(ncells_)); HIPCHK(cs_.alloc(ncells_ + 1));
     HIPCHK(flag_.alloc(1));
     // Neighbour-list capacity: 1.5x the mean number of particles inside the
-    // cut-off sphere (disc in 2D) at the initial number density, plus 64.
-    {
-      double vol = P_.L[0] * (P_.L[1] + (P_.wall ? 2.0 * P_.wall_width : 0.0)) *
-                   (P_.dim == 3 ? P_.L[2] : 1.0);
-      double n0 = N_ / vol;
-      double nb = P_.dim == 3 ? n0 * 4.0 / 3.0 * M_PI * P_.rcut * P_.rcut * P_.rcut
-                              : n0 * M_PI * P_.rcut * P_.rcut;
-      kmax_ = ((int)(1.5 * nb) + 64 + 31) / 32 * 32;
-      // SPH_NEIGHBOUR_CAPACITY overrides it (smaller lists save memory; the
-      // pair pass re-scans for particles whose neighbours do not fit).
-      if (const char *e = std::getenv("SPH_NEIGHBOUR_CAPACITY")) kmax_ = std::max(1, std::atoi(e));
+    // cut-off sphere (disc in 2D) at the initial number density, plus 64, for
+    // the densest simulation of the batch.
+    kmax_ = 4;
+    for (const SimInfo &si : sims_) {
+      const sph_params_t &p = si.p;
+      double vol = p.L[0] * (p.L[1] + (p.wall ? 2.0 * p.wall_width : 0.0)) *
+                   (p.dim == 3 ? p.L[2] : 1.0);
+      double n0 = p.N / vol;
+      double nb = p.dim == 3 ? n0 * 4.0 / 3.0 * M_PI * p.rcut * p.rcut * p.rcut
+                             : n0 * M_PI * p.rcut * p.rcut;
+      kmax_ = std::max(kmax_, ((int)(1.5 * nb) + 64 + 31) / 32 * 32);
     }
+    // SPH_NEIGHBOUR_CAPACITY overrides it (smaller lists save memory; the pair
+    // pass re-scans for particles whose neighbours do not fit).
+    if (const char *e = std::getenv("SPH_NEIGHBOUR_CAPACITY")) kmax_ = std::max(1, std::atoi(e));
     kmax_ = (kmax_ + 3) / 4 * 4;
     HIPCHK(nl_idx_.alloc((size_t)(kmax_ / 4) * N_));
     HIPCHK(nl_count_.alloc(N_));
@@ -1358,41 +1514,59 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemset(nl_qlen_.p, 0, sizeof(int)));
     HIPCHK(dinv_.alloc(N_));
     HIPCHK(rec_.alloc(N_));
-    HIPCHK(ke_part_.alloc(kKeBlocks)); HIPCHK(ke_out_.alloc(1));
-    HIPCHK(obs_part_.alloc(3 * kKeBlocks)); HIPCHK(obs_out_.alloc(6));
-    const int Nc = P_.N_colloids;
-    for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(Nc));
-    HIPCHK(clist_.alloc(P_.Nlist_colloids)); HIPCHK(cstart_.alloc(Nc + 1));
-    HIPCHK(cowner_.alloc(P_.Nlist_colloids));
-    HIPCHK(xc_.alloc(P_.Nlist_colloids)); HIPCHK(yc_.alloc(P_.Nlist_colloids));
-    HIPCHK(zc_.alloc(P_.Nlist_colloids));
-    HIPCHK(wlist_.alloc(P_.Nlist_walls)); HIPCHK(wstart_.alloc(std::max(P_.Nwalls, 1)));
-    HIPCHK(fwall_.alloc(6));
+    HIPCHK(ke_part_.alloc((size_t)kKeBlocks * S)); HIPCHK(ke_out_.alloc(S));
+    HIPCHK(obs_part_.alloc((size_t)3 * kKeBlocks * S)); HIPCHK(obs_out_.alloc(6 * S));
+    for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(ncoll_));
+    HIPCHK(clist_.alloc(nclist_)); HIPCHK(cstart_.alloc(ncoll_ + 1));
+    HIPCHK(cowner_.alloc(nclist_));
+    HIPCHK(xc_.alloc(nclist_)); HIPCHK(yc_.alloc(nclist_)); HIPCHK(zc_.alloc(nclist_));
+    HIPCHK(wlist_.alloc(nwlist_));
+    HIPCHK(fwall_.alloc(6 * S));
+    HIPCHK(steps_.alloc(S)); HIPCHK(mag_.alloc(3 * S));
 
-    // particle state, ID order = initial order
-    for (int k : {F_X, F_Y, F_Z}) {
-      DevBuf<double> &dst = k == F_X ? x_[0] : (k == F_Y ? y_[0] : z_[0]);
-      HIPCHK(hipMemcpy(dst.p, hs.pf[k].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
-    }
-    int rc = put_real(v_[0][0].p, hs.pf[F_VX], nullptr, err); if (rc) return rc;
-    rc = put_real(v_[0][1].p, hs.pf[F_VY], nullptr, err); if (rc) return rc;
-    rc = put_real(v_[0][2].p, hs.pf[F_VZ], nullptr, err); if (rc) return rc;
-    static const int cfield[6] = {F_CXX, F_CXY, F_CYY, F_CXZ, F_CYZ, F_CZZ};
-    for (int a = 0; a < 6; ++a) {
-      rc = put_real(c_[0][a].p, hs.pf[cfield[a]], nullptr, err); if (rc) return rc;
-    }
-    rc = put_real(mass_[0].p, hs.pf[F_MASS], nullptr, err); if (rc) return rc;
-    HIPCHK(hipMemcpy(type_[0].p, hs.type.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
-    std::vector<int> iota(N_);
+    // particle state, ID order = initial order; IDs are global (pbeg + local)
+    std::vector<double> xyz[3];
+    std::vector<R> real(N_);
+    for (int a = 0; a < 3; ++a) xyz[a].resize(N_);
+    std::vector<int> ityp(N_), iota(N_);
     std::iota(iota.begin(), iota.end(), 0);
+    for (int s = 0; s < S; ++s) {
+      const HostState &h = *hs[s];
+      const int b = sims_[s].pbeg;
+      for (int i = 0; i < h.p.N; ++i) {
+        xyz[0][b + i] = h.pf[F_X][i];
+        xyz[1][b + i] = h.pf[F_Y][i];
+        xyz[2][b + i] = h.pf[F_Z][i];
+        ityp[b + i] = h.type[i];
+      }
+    }
+    HIPCHK(hipMemcpy(x_[0].p, xyz[0].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(y_[0].p, xyz[1].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(z_[0].p, xyz[2].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+    auto put_all = [&](R *dst, int field) -> int {
+      for (int s = 0; s < S; ++s) {
+        const HostState &h = *hs[s];
+        for (int i = 0; i < h.p.N; ++i) real[sims_[s].pbeg + i] = (R)h.pf[field][i];
+      }
+      HIPCHK(hipMemcpy(dst, real.data(), N_ * sizeof(R), hipMemcpyHostToDevice));
+      return SPH_OK;
+    };
+    int rc;
+    if ((rc = put_all(v_[0][0].p, F_VX)) || (rc = put_all(v_[0][1].p, F_VY)) ||
+        (rc = put_all(v_[0][2].p, F_VZ)))
+      return rc;
+    static const int cfield[6] = {F_CXX, F_CXY, F_CYY, F_CXZ, F_CYZ, F_CZZ};
+    for (int a = 0; a < 6; ++a)
+      if ((rc = put_all(c_[0][a].p, cfield[a]))) return rc;
+    if ((rc = put_all(mass_[0].p, F_MASS))) return rc;
+    HIPCHK(hipMemcpy(type_[0].p, ityp.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(id_[0].p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(rank_.p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
     id_host_ = iota;
     // derived fields start at zero (reference: cudaMemset, system_copy_pointers_to_device.cpp:190-256)
     static const int ffield[3] = {F_FX, F_FY, F_FZ};
-    for (int a = 0; a < 3; ++a) {
-      rc = put_real(f_[a].p, hs.pf[ffield[a]], nullptr, err); if (rc) return rc;
-    }
+    for (int a = 0; a < 3; ++a)
+      if ((rc = put_all(f_[a].p, ffield[a]))) return rc;
     for (int a = 0; a < 6; ++a) HIPCHK(hipMemset(dcdt_[a].p, 0, N_ * sizeof(R)));
     for (int a = 0; a < 9; ++a) HIPCHK(hipMemset(gv_[a].p, 0, N_ * sizeof(R)));
     // The second buffer set starts as a copy, so rows a failed reorder leaves
@@ -1409,45 +1583,98 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemcpy(id_[1].p, id_[0].p, N_ * sizeof(int), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemset(dens_.p, 0, N_ * sizeof(R)));
     HIPCHK(hipMemset(press_.p, 0, N_ * sizeof(R)));
-    HIPCHK(hipMemset(count_.p, 0, P_.Ntotal_cells * sizeof(int)));
-    HIPCHK(hipMemset(cs_.p, 0, (P_.Ntotal_cells + 1) * sizeof(int)));
+    HIPCHK(hipMemset(count_.p, 0, ncells_ * sizeof(int)));
+    HIPCHK(hipMemset(cs_.p, 0, (ncells_ + 1) * sizeof(int)));
     HIPCHK(hipMemset(flag_.p, 0, sizeof(int)));
-    HIPCHK(hipMemset(fwall_.p, 0, 6 * sizeof(double)));
-    for (int k = 0; k < C_NUM; ++k)
-      if (Nc) HIPCHK(hipMemcpy(coll_[k].p, hs.cf[k].data(), Nc * sizeof(double),
-                               hipMemcpyHostToDevice));
-    if (P_.Nlist_colloids) {
-      HIPCHK(hipMemcpy(clist_.p, hs.colloids_list.data(), P_.Nlist_colloids * sizeof(int),
-                       hipMemcpyHostToDevice));
-      std::vector<int> owner(P_.Nlist_colloids);
-      std::vector<int> cst(Nc + 1);
-      for (int i = 0; i < Nc; ++i) cst[i] = hs.colloids_start[i];
-      cst[Nc] = P_.Nlist_colloids;
-      for (int i = 0; i < Nc; ++i)
-        for (int j = cst[i]; j < cst[i + 1]; ++j) owner[j] = i;
-      HIPCHK(hipMemcpy(cstart_.p, cst.data(), (Nc + 1) * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(cowner_.p, owner.data(), P_.Nlist_colloids * sizeof(int),
-                       hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(xc_.p, hs.x_center.data(), P_.Nlist_colloids * sizeof(double),
-                       hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(yc_.p, hs.y_center.data(), P_.Nlist_colloids * sizeof(double),
-                       hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(zc_.p, hs.z_center.data(), P_.Nlist_colloids * sizeof(double),
-                       hipMemcpyHostToDevice));
-    } else if (Nc) {
-      std::vector<int> cst(Nc + 1, 0);
-      HIPCHK(hipMemcpy(cstart_.p, cst.data(), (Nc + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(fwall_.p, 0, 6 * S * sizeof(double)));
+    HIPCHK(hipMemset(steps_.p, 0, S * sizeof(int)));
+    HIPCHK(hipMemset(mag_.p, 0, 3 * S * sizeof(double)));
+
+    // colloids and lists, concatenated; list entries hold global particle IDs
+    std::vector<double> cf(ncoll_);
+    for (int k = 0; k < C_NUM; ++k) {
+      for (int s = 0; s < S; ++s)
+        for (int c = 0; c < hs[s]->p.N_colloids; ++c) cf[sims_[s].coll_off + c] = hs[s]->cf[k][c];
+      if (ncoll_)
+        HIPCHK(hipMemcpy(coll_[k].p, cf.data(), ncoll_ * sizeof(double), hipMemcpyHostToDevice));
     }
-    if (P_.Nlist_walls)
-      HIPCHK(hipMemcpy(wlist_.p, hs.walls_list.data(), P_.Nlist_walls * sizeof(int),
-                       hipMemcpyHostToDevice));
-    if (P_.Nwalls)
-      HIPCHK(hipMemcpy(wstart_.p, hs.walls_start.data(), P_.Nwalls * sizeof(int),
-                       hipMemcpyHostToDevice));
+    std::vector<int> clist(nclist_), cowner(nclist_), cst(ncoll_ + 1), csim(ncoll_),
+        coff(S + 1), pbeg(S + 1), coffc(S), wlist(nwlist_), wr(2 * S + 1);
+    std::vector<double> xc(nclist_), yc(nclist_), zc(nclist_);
+    for (int s = 0; s < S; ++s) {
+      const HostState &h = *hs[s];
+      const SimInfo &si = sims_[s];
+      const int nc = h.p.N_colloids;
+      for (int c = 0; c < nc; ++c) {
+        const int g = si.coll_off + c;
+        csim[g] = s;
+        cst[g] = si.clist_off + (h.p.Nlist_colloids ? h.colloids_start[c] : 0);
+        const int e = c + 1 < nc ? h.colloids_start[c + 1] : h.p.Nlist_colloids;
+        for (int j = h.colloids_start[c]; j < e; ++j) cowner[si.clist_off + j] = g;
+      }
+      for (int j = 0; j < h.p.Nlist_colloids; ++j) {
+        clist[si.clist_off + j] = si.pbeg + h.colloids_list[j];
+        xc[si.clist_off + j] = h.x_center[j];
+        yc[si.clist_off + j] = h.y_center[j];
+        zc[si.clist_off + j] = h.z_center[j];
+      }
+      const int nw = h.p.wall ? h.p.Nlist_walls : 0;
+      for (int j = 0; j < nw; ++j) wlist[si.wlist_off + j] = si.pbeg + h.walls_list[j];
+      wr[2 * s] = si.wlist_off + (nw && h.p.Nwalls > 0 ? h.walls_start[0] : 0);
+      wr[2 * s + 1] = si.wlist_off + (nw && h.p.Nwalls > 1 ? h.walls_start[1] : nw);
+      coff[s] = si.coll_off;
+      pbeg[s] = si.pbeg;
+      coffc[s] = si.cell_off;
+    }
+    cst[ncoll_] = nclist_;
+    coff[S] = ncoll_;
+    pbeg[S] = N_;
+    wr[2 * S] = nwlist_;
+    auto up = [&](DevBuf<int> &d, const std::vector<int> &v) -> int {
+      HIPCHK(d.alloc(v.size()));
+      if (!v.empty())
+        HIPCHK(hipMemcpy(d.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+      return SPH_OK;
+    };
+    if ((rc = up(pbeg_, pbeg)) || (rc = up(cell_off_, coffc)) || (rc = up(coll_off_, coff)) ||
+        (rc = up(coll_sim_, csim)) || (rc = up(wrange_, wr)))
+      return rc;
+    if (nclist_) {
+      HIPCHK(hipMemcpy(clist_.p, clist.data(), nclist_ * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(cowner_.p, cowner.data(), nclist_ * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(xc_.p, xc.data(), nclist_ * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(yc_.p, yc.data(), nclist_ * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(zc_.p, zc.data(), nclist_ * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(cstart_.p, cst.data(), (ncoll_ + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (nwlist_)
+      HIPCHK(hipMemcpy(wlist_.p, wlist.data(), nwlist_ * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<sph_params_t> ps(S);
+    for (int s = 0; s < S; ++s) ps[s] = sims_[s].p;
+    HIPCHK(simP_.alloc(S));
+    HIPCHK(hipMemcpy(simP_.p, ps.data(), S * sizeof(sph_params_t), hipMemcpyHostToDevice));
+    if ((rc = build_blocks(err))) return rc;
     cur_ = 0;
     have_cells_ = false;
     HIPCHK(hipDeviceSynchronize());
     return SPH_OK;
+  }
+
+  int set_launch(int block, int xcd, std::string &err) override {
+    if (block != 64 && block != 128 && block != 256) {
+      err = "block must be 64, 128 or 256 threads";
+      return SPH_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    block_ = block;
+    xcd_ = xcd ? 1 : 0;
+    drop_graphs();
+    return sims_.empty() ? SPH_OK : build_blocks(err);
+  }
+
+  void get_launch(int *block, int *xcd) const override {
+    *block = block_;
+    *xcd = xcd_;
   }
 
   // ---- per-step phases ----
@@ -1455,13 +1682,13 @@ class DeviceImpl final : public Device {
     Mark m(this, PH_NS);
     const int nb = blocks(N_);
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
-    k_hash<<<nb, kBlock, 0, stream_>>>(P_, S.x, S.y, S.z, cell_.p, slot_.p, count_.p, flag_.p, N_);
-    k_scan_cells<<<1, 1024, 0, stream_>>>(count_.p, cs_.p, P_.Ntotal_cells);
+    k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
+                                          flag_.p);
+    k_scan_cells<<<1, 1024, 0, stream_>>>(count_.p, cs_.p, ncells_);
     k_scatter<<<nb, kBlock, 0, stream_>>>(cell_.p, slot_.p, cs_.p, S.id, stage_src_.p,
                                           stage_id_.p, N_);
     k_rank_gather<R><<<nb, kBlock, 0, stream_>>>(S, T, cell_.p, cs_.p, stage_src_.p, stage_id_.p,
-                                                 rank_.p, sorted_cell_.p, P_.Ntotal_cells, nc_,
-                                                 N_);
+                                                 rank_.p, sorted_cell_.p, ncells_, nc_, N_);
     HIPCHK(hipGetLastError());
     cur_ ^= 1;
     list_valid_ = false;
@@ -1479,8 +1706,8 @@ class DeviceImpl final : public Device {
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_density<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), rec_.p, cs_.p, nlist(), N_, write_dens);
+      k_density<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(),
+                                                              rec_.p, cs_.p, nlist(), write_dens);
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
@@ -1494,7 +1721,7 @@ class DeviceImpl final : public Device {
 
   // calculate_pressure: pressure, 1/d and the pair records from the current densities.
   int pressure(std::string &err) override {
-    k_pressure<R><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), rec_.p, N_);
+    k_pressure<R><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(), rec_.p);
     HIPCHK(hipGetLastError());
     rec_valid_ = true;
     return SPH_OK;
@@ -1511,12 +1738,12 @@ class DeviceImpl final : public Device {
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_pairs<R, DIM, WALL, GV, F><<<blocks(N_), kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), rec_.p, nlist(), N_, store_gv, write_dcdt,
+      k_pairs<R, DIM, WALL, GV, F><<<nblk_, block_, 0, stream_>>>(
+          table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), store_gv, write_dcdt,
           with_ext);
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
-          P_, parts(cur_), derived(), bodies(), rec_.p, nlist(), cs_.p, store_gv, write_dcdt,
-          with_ext);
+          table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), cs_.p, store_gv,
+          write_dcdt, with_ext);
     });
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1537,30 +1764,38 @@ class DeviceImpl final : public Device {
   }
 
   int external_forces(std::string &err) override {
-    if (P_.ext_force_type == 0) return SPH_OK;
     if (P_.dim == 3)
-      k_ext<R, 3><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+      k_ext<R, 3><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
     else
-      k_ext<R, 2><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+      k_ext<R, 2><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
 
   int force_walls(std::string &err) override {
-    if (!P_.wall || P_.Nwalls == 0) return SPH_OK;
+    if (!any_walls_) return SPH_OK;
     Mark m(this, PH_BODIES);
-    k_walls<R><<<P_.Nwalls, kRedBlock, 0, stream_>>>(P_, derived(), bodies(), rank_.p);
+    k_walls<R><<<dim3(2, nsim()), kRedBlock, 0, stream_>>>(table(), derived(), bodies(), rank_.p);
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
 
   int force_colloids(std::string &err) override {
-    if (P_.N_colloids == 0) return SPH_OK;
+    if (ncoll_ == 0) return SPH_OK;
     Mark m(this, PH_BODIES);
-    k_colloids<R><<<P_.N_colloids, kBlock, 0, stream_>>>(P_, parts(cur_), derived(), bodies(),
-                                                         rank_.p, mag_[0], mag_[1], mag_[2]);
+    k_colloids<R><<<ncoll_, kBlock, 0, stream_>>>(table(), parts(cur_), derived(), bodies(),
+                                                  rank_.p, mag_.p);
     HIPCHK(hipGetLastError());
     return SPH_OK;
+  }
+
+  // calculate_forces as one sequence (src/system_calculate_forces.cu:86-170).
+  int forces_all(bool store_gradv, std::string &err) override {
+    int rc;
+    if ((rc = density(err))) return rc;
+    if ((rc = pairs_all(store_gradv, err))) return rc;
+    if ((rc = force_walls(err))) return rc;
+    return force_colloids(err);
   }
 
   int vv1(std::string &err) override {
@@ -1570,13 +1805,13 @@ class DeviceImpl final : public Device {
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_vv1<R, DIM, WALL><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+      k_vv1<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
     });
-    if (P_.N_colloids > 0 && P_.coll_move == 0) {
-      k_coll_vv1<<<blocks(P_.N_colloids), kBlock, 0, stream_>>>(P_, bodies());
-      if (P_.Nlist_colloids > 0)
-        k_coll_place<R><<<blocks(P_.Nlist_colloids), kBlock, 0, stream_>>>(
-            P_, parts(cur_), bodies(), rank_.p, P_.Nlist_colloids);
+    if (ncoll_ > 0) {
+      k_coll_vv1<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
+      if (nclist_ > 0)
+        k_coll_place<R><<<blocks(nclist_), kBlock, 0, stream_>>>(table(), parts(cur_), bodies(),
+                                                                 rank_.p, nclist_);
     }
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1586,41 +1821,103 @@ class DeviceImpl final : public Device {
     Mark m(this, PH_VV2);
     rec_valid_ = false;
     if (P_.dim == 3)
-      k_vv2<R, 3><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
+      k_vv2<R, 3><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
     else
-      k_vv2<R, 2><<<blocks(N_), kBlock, 0, stream_>>>(P_, parts(cur_), derived(), N_);
-    if (P_.coll_move == 0 && P_.N_colloids > 0)
-      k_coll_vv2<<<blocks(P_.N_colloids), kBlock, 0, stream_>>>(P_, bodies());
+      k_vv2<R, 2><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
+    if (ncoll_ > 0) k_coll_vv2<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
 
-  void set_magnetic_moment(const double m[3]) override {
-    mag_[0] = m[0];
-    mag_[1] = m[1];
-    mag_[2] = m[2];
+  int set_step(int step, std::string &err) override {
+    if (ncoll_ == 0) return SPH_OK;  // the moment only enters the colloid forces
+    k_set_step<<<blocks(nsim()), kBlock, 0, stream_>>>(table(), steps_.p, mag_.p, step);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
   }
 
-  int observable_sums(double out[6], std::string &err) override {
-    k_polymer_partial<R><<<kKeBlocks, kBlock, 0, stream_>>>(parts(cur_), obs_part_.p, N_);
-    k_observables_final<<<1, kBlock, 0, stream_>>>(P_, bodies(), obs_part_.p, kKeBlocks,
+  // One step in the order of src/main.cu:361-459.  `step` >= 0 sets the step
+  // counter, -1 advances the device counter (captured form).
+  int one_step(int step, std::string &err) {
+    int rc;
+    if ((rc = vv1(err))) return rc;
+    if ((rc = neighbours_search(err))) return rc;
+    if ((rc = set_step(step, err))) return rc;
+    if ((rc = forces_all(false, err))) return rc;
+    return vv2(err);
+  }
+
+  int run_steps(int nsteps, int first_step, bool graph, std::string &err) override {
+    int rc;
+    if (!graph || timing_) {
+      for (int n = 0; n < nsteps; ++n)
+        if ((rc = one_step(first_step + n, err))) return rc;
+      return SPH_OK;
+    }
+    if (nsteps > 0 && (rc = set_step(first_step - 1, err))) return rc;
+    for (int n = 0; n < nsteps; ++n) {
+      // The reorder flips the buffer set, so the captured launches differ by
+      // the parity of the current set: one graph per parity.
+      const int par = cur_;
+      if (!exec_[par]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        rc = one_step(-1, err);
+        hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess) {
+          err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+          return SPH_ERR_DEVICE;
+        }
+        e = hipGraphInstantiate(&exec_[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          exec_[par] = nullptr;
+          err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+          return SPH_ERR_DEVICE;
+        }
+        cur_ = par;  // capture recorded the launches without running them
+      }
+      HIPCHK(hipGraphLaunch(exec_[par], stream_));
+      cur_ = par ^ 1;
+    }
+    // state after a step: positions unchanged since the density pass, velocities moved
+    list_valid_ = true;
+    rec_valid_ = false;
+    have_cells_ = true;
+    id_dirty_ = true;
+    return SPH_OK;
+  }
+
+  int observable_sums(double *out, std::string &err) override {
+    const int S = nsim();
+    k_polymer_partial<R><<<dim3(kKeBlocks, S), kBlock, 0, stream_>>>(table(), parts(cur_),
+                                                                     obs_part_.p);
+    k_observables_final<<<S, kBlock, 0, stream_>>>(table(), bodies(), obs_part_.p, kKeBlocks,
                                                    obs_out_.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, obs_out_.p, 6 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(out, obs_out_.p, 6 * S * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     return SPH_OK;
   }
 
   int kinetic_energy(double *ke, std::string &err) override {
-    k_ke_partial<R><<<kKeBlocks, kBlock, 0, stream_>>>(P_, parts(cur_), ke_part_.p, N_);
-    k_ke_final<<<1, kBlock, 0, stream_>>>(ke_part_.p, kKeBlocks, ke_out_.p);
+    const int S = nsim();
+    k_ke_partial<R><<<dim3(kKeBlocks, S), kBlock, 0, stream_>>>(table(), parts(cur_),
+                                                                ke_part_.p);
+    k_ke_final<<<S, kBlock, 0, stream_>>>(ke_part_.p, kKeBlocks, ke_out_.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ke, ke_out_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(ke, ke_out_.p, S * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    if (!std::isfinite(*ke)) {
-      err = "kinetic energy is not finite";
-      return SPH_ERR_NAN;
-    }
+    for (int s = 0; s < S; ++s)
+      if (!std::isfinite(ke[s])) {
+        err = "kinetic energy is not finite";
+        return SPH_ERR_NAN;
+      }
     return SPH_OK;
   }
 
@@ -1643,45 +1940,52 @@ class DeviceImpl final : public Device {
   }
 
   // ---- field exchange (ID order on the host side) ----
-  int download_field(HostState &hs, const std::string &name, std::string &err) override {
+  int download_field(int sim, HostState &hs, const std::string &name,
+                     std::string &err) override {
     HIPCHK(hipStreamSynchronize(stream_));
     int rc = refresh_ids(err);
     if (rc) return rc;
+    const SimInfo &si = sims_[sim];
+    const int b = si.pbeg, n = si.n;
     for (int k = 0; k < F_NUM; ++k) {
       if (name != kPFieldNames[k]) continue;
       std::vector<double> &dst = hs.pf[k];
-      dst.assign(std::max(hs.p.Nmax, N_), 0.0);
+      dst.assign(std::max(hs.p.Nmax, n), 0.0);
       if (k == F_X || k == F_Y || k == F_Z) {
-        std::vector<double> tmp(N_);
+        std::vector<double> tmp(n);
         const double *src = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
-        HIPCHK(hipMemcpy(tmp.data(), src, N_ * sizeof(double), hipMemcpyDeviceToHost));
-        for (int p = 0; p < N_; ++p) dst[id_host_[p]] = tmp[p];
+        HIPCHK(hipMemcpy(tmp.data(), src + b, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < n; ++p)
+          if (const int l = local_id(b, n, p); l >= 0) dst[l] = tmp[p];
         return SPH_OK;
       }
       const R *src = real_ptr(k);
-      std::vector<R> tmp(N_);
-      HIPCHK(hipMemcpy(tmp.data(), src, N_ * sizeof(R), hipMemcpyDeviceToHost));
-      for (int p = 0; p < N_; ++p) dst[id_host_[p]] = (double)tmp[p];
+      std::vector<R> tmp(n);
+      HIPCHK(hipMemcpy(tmp.data(), src + b, n * sizeof(R), hipMemcpyDeviceToHost));
+      for (int p = 0; p < n; ++p)
+        if (const int l = local_id(b, n, p); l >= 0) dst[l] = (double)tmp[p];
       return SPH_OK;
     }
     if (name == "type") {
-      std::vector<int> tmp(N_);
-      HIPCHK(hipMemcpy(tmp.data(), type_[cur_].p, N_ * sizeof(int), hipMemcpyDeviceToHost));
-      hs.type.assign(std::max(hs.p.Nmax, N_), 0);
-      for (int p = 0; p < N_; ++p) hs.type[id_host_[p]] = tmp[p];
+      std::vector<int> tmp(n);
+      HIPCHK(hipMemcpy(tmp.data(), type_[cur_].p + b, n * sizeof(int), hipMemcpyDeviceToHost));
+      hs.type.assign(std::max(hs.p.Nmax, n), 0);
+      for (int p = 0; p < n; ++p)
+        if (const int l = local_id(b, n, p); l >= 0) hs.type[l] = tmp[p];
       return SPH_OK;
     }
     for (int k = 0; k < C_NUM; ++k) {
       if (name != kCFieldNames[k]) continue;
-      hs.cf[k].assign(P_.N_colloids, 0.0);
-      if (P_.N_colloids)
-        HIPCHK(hipMemcpy(hs.cf[k].data(), coll_[k].p, P_.N_colloids * sizeof(double),
+      const int nc = si.p.N_colloids;
+      hs.cf[k].assign(nc, 0.0);
+      if (nc)
+        HIPCHK(hipMemcpy(hs.cf[k].data(), coll_[k].p + si.coll_off, nc * sizeof(double),
                          hipMemcpyDeviceToHost));
       return SPH_OK;
     }
     if (name == "fx_wall" || name == "fy_wall" || name == "fz_wall") {
       double fw[6];
-      HIPCHK(hipMemcpy(fw, fwall_.p, 6 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(fw, fwall_.p + 6 * sim, 6 * sizeof(double), hipMemcpyDeviceToHost));
       int a = name[1] - 'x';
       hs.fwall[a][0] = fw[a * 2 + 0];
       hs.fwall[a][1] = fw[a * 2 + 1];
@@ -1690,34 +1994,50 @@ class DeviceImpl final : public Device {
     return SPH_OK;  // lists and body offsets never change on the device
   }
 
-  int upload_field(const HostState &hs, const std::string &name, std::string &err) override {
+  int upload_field(int sim, const HostState &hs, const std::string &name,
+                   std::string &err) override {
     HIPCHK(hipStreamSynchronize(stream_));
     rec_valid_ = false;
     int rc = refresh_ids(err);
     if (rc) return rc;
+    const SimInfo &si = sims_[sim];
+    const int b = si.pbeg, n = si.n;
     for (int k = 0; k < F_NUM; ++k) {
       if (name != kPFieldNames[k]) continue;
       if (k == F_X || k == F_Y || k == F_Z) {
-        std::vector<double> tmp(N_);
-        for (int p = 0; p < N_; ++p) tmp[p] = hs.pf[k][id_host_[p]];
+        std::vector<double> tmp(n);
+        for (int p = 0; p < n; ++p) {
+          const int l = local_id(b, n, p);
+          tmp[p] = l >= 0 ? hs.pf[k][l] : 0.0;
+        }
         double *dst = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
-        HIPCHK(hipMemcpy(dst, tmp.data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dst + b, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
         list_valid_ = false;
         return SPH_OK;
       }
-      return put_real(real_ptr(k), hs.pf[k], &id_host_, err);
+      std::vector<R> tmp(n);
+      for (int p = 0; p < n; ++p) {
+        const int l = local_id(b, n, p);
+        tmp[p] = l >= 0 ? (R)hs.pf[k][l] : R(0);
+      }
+      HIPCHK(hipMemcpy(real_ptr(k) + b, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice));
+      return SPH_OK;
     }
     if (name == "type") {
       list_valid_ = false;
-      std::vector<int> tmp(N_);
-      for (int p = 0; p < N_; ++p) tmp[p] = hs.type[id_host_[p]];
-      HIPCHK(hipMemcpy(type_[cur_].p, tmp.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
+      std::vector<int> tmp(n);
+      for (int p = 0; p < n; ++p) {
+        const int l = local_id(b, n, p);
+        tmp[p] = l >= 0 ? hs.type[l] : 0;
+      }
+      HIPCHK(hipMemcpy(type_[cur_].p + b, tmp.data(), n * sizeof(int), hipMemcpyHostToDevice));
       return SPH_OK;
     }
     for (int k = 0; k < C_NUM; ++k) {
       if (name != kCFieldNames[k]) continue;
-      if (P_.N_colloids)
-        HIPCHK(hipMemcpy(coll_[k].p, hs.cf[k].data(), P_.N_colloids * sizeof(double),
+      const int nc = si.p.N_colloids;
+      if (nc)
+        HIPCHK(hipMemcpy(coll_[k].p + si.coll_off, hs.cf[k].data(), nc * sizeof(double),
                          hipMemcpyHostToDevice));
       return SPH_OK;
     }
@@ -1725,20 +2045,30 @@ class DeviceImpl final : public Device {
     return SPH_ERR_ARG;
   }
 
-  int cell_list(int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce, std::string &err) override {
+  int cell_list(int sim, int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce,
+                std::string &err) override {
     if (!have_cells_) {
       err = "no neighbours_search has run";
       return SPH_ERR_STATE;
     }
     HIPCHK(hipStreamSynchronize(stream_));
-    if (pc) HIPCHK(hipMemcpy(pc, sorted_cell_.p, N_ * sizeof(int), hipMemcpyDeviceToHost));
-    if (pi) HIPCHK(hipMemcpy(pi, id_[cur_].p, N_ * sizeof(int), hipMemcpyDeviceToHost));
+    const SimInfo &si = sims_[sim];
+    const int b = si.pbeg, n = si.n, nc = si.p.Ntotal_cells;
+    if (pc) {
+      HIPCHK(hipMemcpy(pc, sorted_cell_.p + b, n * sizeof(int), hipMemcpyDeviceToHost));
+      for (int p = 0; p < n; ++p) pc[p] -= si.cell_off;
+    }
+    if (pi) {
+      HIPCHK(hipMemcpy(pi, id_[cur_].p + b, n * sizeof(int), hipMemcpyDeviceToHost));
+      for (int p = 0; p < n; ++p) pi[p] -= b;
+    }
     if (cs || ce) {
-      std::vector<int> tmp(P_.Ntotal_cells + 1);
-      HIPCHK(hipMemcpy(tmp.data(), cs_.p, tmp.size() * sizeof(int), hipMemcpyDeviceToHost));
-      for (int c = 0; c < P_.Ntotal_cells; ++c) {
-        if (cs) cs[c] = tmp[c];
-        if (ce) ce[c] = tmp[c + 1] - 1;
+      std::vector<int> tmp(nc + 1);
+      HIPCHK(hipMemcpy(tmp.data(), cs_.p + si.cell_off, tmp.size() * sizeof(int),
+                       hipMemcpyDeviceToHost));
+      for (int c = 0; c < nc; ++c) {
+        if (cs) cs[c] = tmp[c] - b;
+        if (ce) ce[c] = tmp[c + 1] - 1 - b;
       }
     }
     return SPH_OK;
@@ -1781,6 +2111,50 @@ class DeviceImpl final : public Device {
 
   static int blocks(int n) { return std::max(1, (n + kBlock - 1) / kBlock); }
 
+  // Tiles of block_ particles that never straddle two simulations.
+  int build_blocks(std::string &err) {
+    std::vector<int> bs, bf;
+    for (int s = 0; s < nsim(); ++s) {
+      const SimInfo &si = sims_[s];
+      const int nb = std::max(1, (si.n + block_ - 1) / block_);
+      for (int k = 0; k < nb; ++k) {
+        bs.push_back(s);
+        bf.push_back(si.pbeg + k * block_);
+      }
+    }
+    nblk_ = (int)bs.size();
+    HIPCHK(blk_sim_.alloc(nblk_));
+    HIPCHK(blk_first_.alloc(nblk_));
+    HIPCHK(hipMemcpy(blk_sim_.p, bs.data(), nblk_ * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(blk_first_.p, bf.data(), nblk_ * sizeof(int), hipMemcpyHostToDevice));
+    return SPH_OK;
+  }
+
+  void drop_graphs() {
+    for (auto &e : exec_)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
+
+  SimTable table() const {
+    SimTable T;
+    T.P = simP_.p;
+    T.blk_sim = blk_sim_.p;
+    T.blk_first = blk_first_.p;
+    T.pbeg = pbeg_.p;
+    T.cell_off = cell_off_.p;
+    T.coll_off = coll_off_.p;
+    T.coll_sim = coll_sim_.p;
+    T.wrange = wrange_.p;
+    T.nsim = (int)sims_.size();
+    T.nblk = nblk_;
+    T.ntotal = N_;
+    T.xcd = xcd_;
+    return T;
+  }
+
   template <class Fn>
   void dispatch3(Fn &&fn) {
     std::integral_constant<int, 2> d2;
@@ -1817,8 +2191,7 @@ class DeviceImpl final : public Device {
     for (int k = 0; k < C_NUM; ++k) B.s[k] = coll_[k].p;
     B.clist = clist_.p; B.cstart = cstart_.p; B.cowner = cowner_.p;
     B.xc = xc_.p; B.yc = yc_.p; B.zc = zc_.p;
-    B.wlist = wlist_.p; B.wstart = wstart_.p; B.fwall = fwall_.p;
-    B.n_colloids = P_.N_colloids;
+    B.wlist = wlist_.p; B.fwall = fwall_.p;
     return B;
   }
 
@@ -1846,13 +2219,11 @@ class DeviceImpl final : public Device {
     }
   }
 
-  // Writes host values (ID order) into a device array in the current order.
-  int put_real(R *dst, const std::vector<double> &src, const std::vector<int> *ids,
-               std::string &err) {
-    std::vector<R> tmp(N_);
-    for (int p = 0; p < N_; ++p) tmp[p] = (R)src[ids ? (*ids)[p] : p];
-    HIPCHK(hipMemcpy(dst, tmp.data(), N_ * sizeof(R), hipMemcpyHostToDevice));
-    return SPH_OK;
+  // ID within its simulation of the particle at sorted position b + p, or -1
+  // if that slot holds another simulation's particle (after a failed reorder).
+  int local_id(int b, int n, int p) const {
+    const int l = id_host_[b + p] - b;
+    return l >= 0 && l < n ? l : -1;
   }
 
   int refresh_ids(std::string &err) {
@@ -1863,11 +2234,15 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
-  sph_params_t P_{};
-  int N_ = 0, nc_ = 3, cur_ = 0;
+  sph_params_t P_{};  // of simulation 0: dim and wall select the kernel variants
+  std::vector<SimInfo> sims_;
+  int N_ = 0, ncells_ = 0, ncoll_ = 0, nclist_ = 0, nwlist_ = 0;
+  bool any_walls_ = false;
+  int nc_ = 3, cur_ = 0;
+  int block_ = kBlock, xcd_ = 1, nblk_ = 0;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
-  double mag_[3] = {0, 0, 0};
   hipStream_t stream_ = nullptr;
+  hipGraphExec_t exec_[2] = {nullptr, nullptr};
   std::vector<Ev> events_;
   std::vector<int> id_host_;
 
@@ -1883,28 +2258,13 @@ class DeviceImpl final : public Device {
   int kmax_ = 0;
   bool list_valid_ = false;
   DevBuf<double> ke_part_, ke_out_, obs_part_, obs_out_;
-  DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_;
-  DevBuf<int> clist_, cstart_, cowner_, wlist_, wstart_;
-
- public:
-  // calculate_forces as one sequence (src/system_calculate_forces.cu:86-170).
-  int forces_all(bool store_gradv, std::string &err) {
-    int rc;
-    if ((rc = density(err))) return rc;
-    if ((rc = pairs_all(store_gradv, err))) return rc;
-    if ((rc = force_walls(err))) return rc;
-    return force_colloids(err);
-  }
+  DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_, mag_;
+  DevBuf<int> clist_, cstart_, cowner_, wlist_, steps_;
+  DevBuf<sph_params_t> simP_;
+  DevBuf<int> blk_sim_, blk_first_, pbeg_, cell_off_, coll_off_, coll_sim_, wrange_;
 };
 
 }  // namespace
-
-// Exposed for the C-ABI layer: the fused calculate_forces sequence.
-int device_forces_all(Device *d, int precision, bool store_gradv, std::string &err) {
-  if (precision == SPH_FP32)
-    return static_cast<DeviceImpl<float> *>(d)->forces_all(store_gradv, err);
-  return static_cast<DeviceImpl<double> *>(d)->forces_all(store_gradv, err);
-}
 
 std::unique_ptr<Device> make_device(int precision) {
   if (precision == SPH_FP32) return std::unique_ptr<Device>(new DeviceImpl<float>());

@@ -36,6 +36,11 @@ ENTRY_POINTS = [
     "sph_V_Verlet_step2", "sph_set_step", "sph_prepare", "sph_step", "sph_synchronize",
     "sph_current_step", "sph_run", "sph_kinetic_energy", "sph_print_output",
     "sph_write_restart", "sph_set_timing", "sph_get_timing", "sph_observables",
+    "sph_batch_create", "sph_batch_destroy", "sph_batch_last_error", "sph_batch_add",
+    "sph_batch_size", "sph_batch_prepare", "sph_batch_step", "sph_batch_synchronize",
+    "sph_batch_current_step", "sph_batch_get_field", "sph_batch_kinetic_energy",
+    "sph_batch_observables", "sph_batch_set_launch", "sph_batch_get_launch",
+    "sph_batch_autotune",
 ]
 OBSERVABLES = ["kinetic_energy", "polymer_N1", "polymer_shear", "wall_shear_bottom",
                "wall_shear_top", "chain_angle", "field_angle", "phase_lag"]
@@ -154,6 +159,31 @@ def library(path=None):
     lib.sph_observables.argtypes = [vp, vp]
     lib.sph_get_timing.restype = i32
     lib.sph_get_timing.argtypes = [vp, vp, vp]
+    lib.sph_batch_create.restype = vp
+    lib.sph_batch_create.argtypes = [i32]
+    lib.sph_batch_destroy.restype = None
+    lib.sph_batch_destroy.argtypes = [vp]
+    lib.sph_batch_last_error.restype = cp
+    lib.sph_batch_last_error.argtypes = [vp]
+    for fn in ("sph_batch_size", "sph_batch_prepare", "sph_batch_synchronize",
+               "sph_batch_current_step"):
+        getattr(lib, fn).restype = i32
+        getattr(lib, fn).argtypes = [vp]
+    lib.sph_batch_add.restype = i32
+    lib.sph_batch_add.argtypes = [vp, vp, vp]
+    lib.sph_batch_step.restype = i32
+    lib.sph_batch_step.argtypes = [vp, i32]
+    lib.sph_batch_get_field.restype = i32
+    lib.sph_batch_get_field.argtypes = [vp, i32, cp, vp, i64]
+    for fn in ("sph_batch_kinetic_energy", "sph_batch_observables"):
+        getattr(lib, fn).restype = i32
+        getattr(lib, fn).argtypes = [vp, vp]
+    lib.sph_batch_set_launch.restype = i32
+    lib.sph_batch_set_launch.argtypes = [vp, i32, i32]
+    lib.sph_batch_get_launch.restype = i32
+    lib.sph_batch_get_launch.argtypes = [vp, vp, vp]
+    lib.sph_batch_autotune.restype = i32
+    lib.sph_batch_autotune.argtypes = [vp, i32, vp, vp, vp]
     if path is None:
         _LIB = lib
     return lib
@@ -341,3 +371,112 @@ class System:
         calls = np.zeros(8, np.int64)
         self._check(self.lib.sph_get_timing(self.h, _ptr(ms), _ptr(calls)), "get_timing")
         return {PHASES[k]: (float(ms[k]), int(calls[k])) for k in range(8)}
+
+
+class Batch:
+    """Many independent simulations in one device state (sph_batch_* in sph.h).
+
+    Add initialized `System`s, then prepare() and step(n): every phase of a
+    step is one launch over all simulations, replayed as a captured graph.
+    Each simulation follows exactly the numbers its own System.step would give.
+    """
+
+    def __init__(self, precision=FP64, lib_path=None):
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "fp32": FP32}[precision.lower()]
+        self.lib = library(lib_path)
+        self.h = self.lib.sph_batch_create(precision)
+        if not self.h:
+            raise ValueError("bad precision")
+        self.precision = precision
+        self.params = []
+
+    def close(self):
+        if self.h:
+            self.lib.sph_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return self.lib.sph_batch_size(self.h)
+
+    def _check(self, rc, call):
+        if rc != 0:
+            msg = self.lib.sph_batch_last_error(self.h).decode(errors="replace")
+            raise SphError(rc, msg, call)
+
+    def add(self, system):
+        idx = ctypes.c_int(-1)
+        self._check(self.lib.sph_batch_add(self.h, system.h, ctypes.byref(idx)), "batch_add")
+        p = SphParams()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(system.params), ctypes.sizeof(p))
+        self.params.append(p)
+        return idx.value
+
+    def prepare(self):
+        self._check(self.lib.sph_batch_prepare(self.h), "batch_prepare")
+
+    def step(self, nsteps=1):
+        self._check(self.lib.sph_batch_step(self.h, int(nsteps)), "batch_step")
+
+    def synchronize(self):
+        self._check(self.lib.sph_batch_synchronize(self.h), "batch_synchronize")
+
+    def current_step(self):
+        return self.lib.sph_batch_current_step(self.h)
+
+    def get(self, sim, name):
+        p = self.params[sim]
+        if name in PARTICLE_FIELDS or name == "type":
+            n = p.N
+        elif name in COLLOID_FIELDS:
+            n = p.N_colloids
+        elif name in WALL_FIELDS:
+            n = p.Nwalls
+        else:
+            raise KeyError(name)
+        out = np.zeros(n, dtype=np.int32 if name in INT_FIELDS else np.float64)
+        self._check(self.lib.sph_batch_get_field(self.h, int(sim), name.encode(), _ptr(out), n),
+                    f"batch get {name}")
+        return out
+
+    def kinetic_energy(self):
+        out = np.zeros(len(self), np.float64)
+        self._check(self.lib.sph_batch_kinetic_energy(self.h, _ptr(out)), "batch_kinetic_energy")
+        return out
+
+    def observables(self):
+        """Array [n_sims, 8] in the order of OBSERVABLES."""
+        out = np.zeros((len(self), 8), np.float64)
+        self._check(self.lib.sph_batch_observables(self.h, _ptr(out)), "batch_observables")
+        return out
+
+    def set_launch(self, block, xcd=True):
+        self._check(self.lib.sph_batch_set_launch(self.h, int(block), 1 if xcd else 0),
+                    "batch_set_launch")
+
+    def get_launch(self):
+        b, x = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.sph_batch_get_launch(self.h, ctypes.byref(b), ctypes.byref(x)),
+                    "batch_get_launch")
+        return b.value, bool(x.value)
+
+    def autotune(self, steps=5):
+        """Times each launch shape over `steps` real steps, keeps the fastest;
+        returns (block, xcd, ms_per_step)."""
+        b, x, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        self._check(self.lib.sph_batch_autotune(self.h, int(steps), ctypes.byref(b),
+                                                ctypes.byref(x), ctypes.byref(ms)),
+                    "batch_autotune")
+        return b.value, bool(x.value), ms.value

@@ -22,8 +22,10 @@ def cfg_colloids_3d(n=30, **kw):
                 omega_magnetic=3.0)
     base.update(kw)
     cfg = inputs.walls_colloids_3d(**base)
-    # Colloids 0 and 1 sit 0.27 apart: inside the repulsion and magnetic ranges.
-    cfg["coll_x"] = [[0.30, 0.45, 0.40], [0.57, 0.50, 0.45], [0.70, 0.30, 0.80]]
+    if "coll_x" not in kw:
+        # Colloids 0 and 1 sit 0.27 apart: inside the repulsion and magnetic ranges.
+        cfg["coll_x"] = [[0.30, 0.45, 0.40], [0.57, 0.50, 0.45], [0.70, 0.30, 0.80]][
+            :cfg["N_colloids"]]
     return cfg
 
 

@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import sph_system
@@ -60,3 +61,30 @@ def test_unknown_field_and_wrong_count(tmp_path):
     assert lib.sph_get_field(s.h, b"nope", buf, 4) == 1
     assert lib.sph_get_field(s.h, b"x", buf, 4) == 1
     assert "entries" in lib.sph_last_error(s.h).decode()
+
+
+def test_batch_host_side_checks(tmp_path):
+    """Batch calls that need no GPU: membership rules and calls before prepare."""
+    import helpers
+    lib = sph_system.library()
+    assert not lib.sph_batch_create(5)
+    b = sph_system.Batch("fp32")
+    with pytest.raises(sph_system.SphError, match="SPH_ERR_STATE"):
+        b.step(1)
+    with pytest.raises(sph_system.SphError, match="empty"):
+        b.prepare()
+    fresh = sph_system.System("fp32")
+    with pytest.raises(sph_system.SphError, match="not initialized"):
+        b.add(fresh)
+    s2 = helpers.make_system(helpers.cfg_couette(n=16), str(tmp_path), noise=0, name="a")
+    s3 = helpers.make_system(helpers.cfg_sine(n=10), str(tmp_path), noise=0, name="b")
+    assert b.add(s2) == 0
+    with pytest.raises(sph_system.SphError, match="share dim and wall"):
+        b.add(s3)
+    assert len(b) == 1
+    with pytest.raises(sph_system.SphError, match="64, 128 or 256"):
+        b.set_launch(100)
+    b.set_launch(128, False)
+    assert b.get_launch() == (128, False)
+    # before prepare the batch serves its host copy
+    np.testing.assert_array_equal(b.get(0, "x"), s2.get("x"))

@@ -181,19 +181,30 @@ def test_bitwise_repeatable(precision, tmp_path):
 
 def test_method_sequence_matches_step(tmp_path):
     """VV1, neighbours_search, set_step, calculate_forces, VV2 called one by one
-    give the same state as the fused sph_step."""
+    give the same state as sph_step, whose steps replay captured graphs (one per
+    buffer parity); checked after every replay, with host reads in between."""
     a = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="a")
     b = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="b")
     a.prepare()
     b.prepare()
-    for step in (1, 2):
+    for step in (1, 2, 3, 4, 5):
         a.V_Verlet_step1()
         a.neighbours_search()
         a.set_step(step)
         a.calculate_forces()
         a.V_Verlet_step2()
-    b.step(2)
-    b.synchronize()
+        b.step(1)
+        b.synchronize()
+        for n in ["x", "y", "vx", "vy", "cxy", "fx", "dens", "coll_x", "tz_colloid"]:
+            np.testing.assert_array_equal(a.get(n), b.get(n), err_msg=f"{n} at step {step}")
+    # several steps in one call
+    for step in (6, 7, 8):
+        a.V_Verlet_step1()
+        a.neighbours_search()
+        a.set_step(step)
+        a.calculate_forces()
+        a.V_Verlet_step2()
+    b.step(3)
     for n in ["x", "y", "vx", "vy", "cxy", "fx", "coll_x", "tz_colloid"]:
         np.testing.assert_array_equal(a.get(n), b.get(n), err_msg=n)
 
