@@ -8,6 +8,13 @@ gradient/dc-dt + forces + wall/colloid forces + VV2, all on the GPU, state
 resident in HBM.  `value` is the FP32 form (pair arithmetic in float,
 positions and reductions in double); the FP64 form is reported beside it.
 
+Also measured (the "sweep" object): the batched parameter sweep - 256
+independent 2D wall + colloid simulations of uneven size (n per side 32-160,
+V, tau, colloid count, field frequency and strength drawn from a seed), FP32
+form, all in one batch whose step is one launch per phase, captured as a
+hipGraph, with the launch shape picked by the autotuner - against the same
+simulations run one after another.
+
 Prints one JSON line.  --gpus above 1 is not supported (one-GPU build).
 """
 import argparse
@@ -24,6 +31,7 @@ sys.path[:0] = [PKG, ORACLE]
 
 import inputs  # noqa: E402
 import sph_system  # noqa: E402
+import sweep  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFLOPS = 157.3
@@ -117,6 +125,47 @@ def cpu_baseline(n, n_colloids, steps, tmpdir):
                 seconds=t1 - t0)
 
 
+def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
+    """Batched sweep throughput vs the same simulations run one at a time."""
+    pts = sweep.sample_points(n_sims, seed=seed)
+    systems = [sweep.build_system(p, steps, sph_system.FP32, tmpdir) for p in pts]
+    n_part = sum(s.N for s in systems)
+    b = sph_system.Batch(sph_system.FP32)
+    for s in systems:
+        b.add(s)
+    b.prepare()
+    block, xcd, _ = b.autotune(steps=3)
+    b.step(5)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    # one simulation at a time: same kernels, one launch set per simulation
+    t_seq = 0.0
+    for s in systems:
+        s.prepare()
+        s.step(2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(seq_steps)
+        s.synchronize()
+        t_seq += time.perf_counter() - t0
+        s.close()
+    batched = n_part * steps / t_batch
+    sequential = n_part * seq_steps / t_seq
+    return {
+        "workload": f"{n_sims} 2D wall + colloid simulations, n per side 32-160 (seed {seed})",
+        "sims": n_sims, "particles": n_part, "steps": steps,
+        "value": batched, "unit": "particle-steps/s",
+        "ms_per_step": 1e3 * t_batch / steps,
+        "launch": {"block": block, "xcd": xcd},
+        "sequential": sequential, "sequential_steps": seq_steps,
+        "batch_speedup": batched / sequential,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +176,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-sims", type=int, default=256)
+    ap.add_argument("--sweep-steps", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
@@ -141,6 +193,12 @@ def main():
             r64 = time_gpu(sph_system.FP64, a.n, a.colloids, a.steps, a.warmup, tmpdir)
             print(f"# fp64: {a.steps} steps in {r64['elapsed']:.3f}s", file=sys.stderr,
                   flush=True)
+        sw = None
+        if not a.no_sweep:
+            sw = time_sweep(a.sweep_sims, a.sweep_steps, 20, tmpdir)
+            print(f"# sweep: {sw['sims']} sims, {sw['particles']} particles, "
+                  f"{sw['ms_per_step']:.3f} ms/step batched, x{sw['batch_speedup']:.1f} vs one "
+                  f"at a time", file=sys.stderr, flush=True)
         cpu = None
         if not a.no_cpu:
             cpu = cpu_baseline(a.n, a.colloids, a.cpu_steps, tmpdir)
@@ -198,6 +256,7 @@ def main():
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,
         },
+        "sweep": sw,
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"],
             "unit": "particle-steps/s",

@@ -222,12 +222,18 @@ int sph_batch_size(sph_batch_handle b);
 int sph_batch_prepare(sph_batch_handle b);
 /* `nsteps` steps of every simulation (as sph_step).  Asynchronous. */
 int sph_batch_step(sph_batch_handle b, int nsteps);
+/* Waits for queued steps.  A simulation that fails (a particle leaves the grid,
+ * or its kinetic energy is not finite) is flagged and does not fail the call:
+ * the other simulations continue unaffected.  flags[s] (sticky) receives
+ * SPH_BATCH_OUT_OF_GRID and/or SPH_BATCH_NAN for simulation s. */
 int sph_batch_synchronize(sph_batch_handle b);
+enum { SPH_BATCH_OUT_OF_GRID = 1, SPH_BATCH_NAN = 2 };
+int sph_batch_failed(sph_batch_handle b, int32_t *flags);
 int sph_batch_current_step(sph_batch_handle b);
 /* Field of simulation `sim`, ID order, names and counts as sph_get_field. */
 int sph_batch_get_field(sph_batch_handle b, int sim, const char *name, void *dst,
                         int64_t count);
-/* ke[s] for every simulation s (as sph_kinetic_energy). */
+/* ke[s] for every simulation s (as sph_kinetic_energy; NaN flags the simulation). */
 int sph_batch_kinetic_energy(sph_batch_handle b, double *ke);
 /* out[8 s + k]: the sph_observables of simulation s. */
 int sph_batch_observables(sph_batch_handle b, double *out);

@@ -33,6 +33,7 @@ struct sph_batch {
   std::string err;
   int step = 0;
   int block = 256, xcd = 1;
+  std::vector<int> failed;  // per simulation, sticky: SPH_BATCH_* bits
 };
 
 namespace {
@@ -282,7 +283,12 @@ int sph_kinetic_energy(sph_handle h, double *ke) {
   if (!h || !ke) return SPH_ERR_ARG;
   int rc = ensure_device(h);
   if (rc) return rc;
-  return h->dev->kinetic_energy(ke, h->err);
+  if ((rc = h->dev->kinetic_energy(ke, h->err))) return rc;
+  if (!std::isfinite(*ke)) {
+    h->err = "kinetic energy is not finite";
+    return SPH_ERR_NAN;
+  }
+  return SPH_OK;
 }
 
 int sph_observables(sph_handle h, double out[8]) {
@@ -290,7 +296,7 @@ int sph_observables(sph_handle h, double out[8]) {
   int rc = ensure_device(h);
   if (rc) return rc;
   double ke = 0.0, s[6];
-  if ((rc = h->dev->kinetic_energy(&ke, h->err))) return rc;
+  if ((rc = sph_kinetic_energy(h, &ke))) return rc;
   if ((rc = h->dev->observable_sums(s, h->err))) return rc;
   if (h->hs.p.wall && (rc = pull(h, {"fx_wall"}))) return rc;
   finish_observables(h->hs.p, h->step, ke, s, h->hs.fwall[0], out);
@@ -438,10 +444,11 @@ int sph_batch_prepare(sph_batch_handle b) {
   if ((rc = dev->set_launch(b->block, b->xcd, b->err))) return rc;
   b->dev = std::move(dev);
   b->step = 0;
+  b->failed.assign(b->sims.size(), 0);
   if ((rc = b->dev->set_step(0, b->err))) return rc;
   if ((rc = b->dev->neighbours_search(b->err))) return rc;
   if ((rc = b->dev->forces_all(true, b->err))) return rc;
-  return b->dev->synchronize(b->err);
+  return sph_batch_synchronize(b);
 }
 
 #define BATCH_READY(b)                               \
@@ -462,9 +469,25 @@ int sph_batch_step(sph_batch_handle b, int nsteps) {
   return SPH_OK;
 }
 
+// Waits for the queued steps and collects the per-simulation flags; a failed
+// simulation does not fail the call (the others are unaffected).
 int sph_batch_synchronize(sph_batch_handle b) {
   BATCH_READY(b);
-  return b->dev->synchronize(b->err);
+  std::vector<int> f;
+  int rc = b->dev->poll_flags(f, b->err);
+  if (rc) return rc;
+  for (size_t s = 0; s < f.size(); ++s)
+    if (f[s]) b->failed[s] |= SPH_BATCH_OUT_OF_GRID;
+  return SPH_OK;
+}
+
+int sph_batch_failed(sph_batch_handle b, int32_t *flags) {
+  BATCH_READY(b);
+  if (!flags) return SPH_ERR_ARG;
+  int rc = sph_batch_synchronize(b);
+  if (rc) return rc;
+  for (size_t s = 0; s < b->failed.size(); ++s) flags[s] = b->failed[s];
+  return SPH_OK;
 }
 
 int sph_batch_current_step(sph_batch_handle b) { return b ? b->step : -1; }
@@ -499,7 +522,11 @@ int sph_batch_get_field(sph_batch_handle b, int sim, const char *name, void *dst
 int sph_batch_kinetic_energy(sph_batch_handle b, double *ke) {
   BATCH_READY(b);
   if (!ke) return SPH_ERR_ARG;
-  return b->dev->kinetic_energy(ke, b->err);
+  int rc = b->dev->kinetic_energy(ke, b->err);
+  if (rc) return rc;
+  for (size_t s = 0; s < b->sims.size(); ++s)
+    if (!std::isfinite(ke[s])) b->failed[s] |= SPH_BATCH_NAN;
+  return SPH_OK;
 }
 
 int sph_batch_observables(sph_batch_handle b, double *out) {
@@ -508,7 +535,7 @@ int sph_batch_observables(sph_batch_handle b, double *out) {
   const int S = (int)b->sims.size();
   std::vector<double> ke(S), sums(6 * S);
   int rc;
-  if ((rc = b->dev->kinetic_energy(ke.data(), b->err))) return rc;
+  if ((rc = sph_batch_kinetic_energy(b, ke.data()))) return rc;
   if ((rc = b->dev->observable_sums(sums.data(), b->err))) return rc;
   for (int s = 0; s < S; ++s) {
     sph::HostState &hs = b->sims[s];

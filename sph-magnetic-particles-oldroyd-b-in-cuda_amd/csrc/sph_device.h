@@ -46,14 +46,16 @@ class Device {
   // captured once per buffer parity and replayed as a hipGraph.
   virtual int run_steps(int nsteps, int first_step, bool graph, std::string &err) = 0;
 
-  // Per simulation: ke[s], and out[6 s + k] as below.
+  // Per simulation: ke[s] (not checked for finiteness), and out[6 s + k] as below.
   virtual int kinetic_energy(double *ke, std::string &err) = 0;
   // [0] sum(cxx-cyy) [1] sum(cxy) over fluid, [2] fluid count, [3] sum sin 2phi,
   // [4] sum cos 2phi, [5] count of colloid pairs within the magnetic cut-off.
   virtual int observable_sums(double *out, std::string &err) = 0;
   virtual int synchronize(std::string &err) = 0;
-  // Checks the device error flag (out-of-grid particle); resets it.
+  // Checks the device error flags (a particle outside the grid); resets them.
   virtual int check_flags(std::string &err) = 0;
+  // The flags per simulation (bit 0: a particle was outside the grid); resets them.
+  virtual int poll_flags(std::vector<int> &flags, std::string &err) = 0;
 
   // Copies a field of simulation `sim` into `hs` (ID order).  `name` as in sph.h.
   virtual int download_field(int sim, HostState &hs, const std::string &name,

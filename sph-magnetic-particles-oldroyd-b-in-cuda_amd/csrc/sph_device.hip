@@ -410,7 +410,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // neighbour search: hash, count, scan, scatter, stable rank + reorder
 // ---------------------------------------------------------------------------
 // kernel_cell_list (src/kernel_cell_list.cu:15-63): clamp c == Ncells for x, z
-// and for y without walls; out-of-grid cells raise the device flag.
+// and for y without walls.  A particle outside the grid (the reference's device
+// assert) raises its simulation's flag and is filed under the nearest cell, so
+// the sorted order stays a permutation within each simulation and the other
+// simulations of a batch are unaffected.
 __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int *cell, int *slot,
@@ -429,18 +432,21 @@ __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__res
     } else {
       cy = (int)floor((y[i] + P.wall_width) / P.cell_size[1]);
     }
+    int cz = 0;
     if (P.dim == 3) {
-      int cz = (int)floor(z[i] / P.cell_size[2]);
+      cz = (int)floor(z[i] / P.cell_size[2]);
       if (cz == P.Ncells[2]) cz = cz - 1;
-      bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] && cz >= 0 &&
-                cz < P.Ncells[2];
-      c = ok ? T.cell_off[sim] + cz * P.Ncells[0] * P.Ncells[1] + cy * P.Ncells[0] + cx : -1;
-    } else {
-      bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1];
-      c = ok ? T.cell_off[sim] + cy * P.Ncells[0] + cx : -1;
     }
+    const bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] &&
+                    (P.dim != 3 || (cz >= 0 && cz < P.Ncells[2]));
+    if (!ok) {  // also catches NaN positions (any int the conversion gives)
+      cx = min(max(cx, 0), P.Ncells[0] - 1);
+      cy = min(max(cy, 0), P.Ncells[1] - 1);
+      cz = P.dim == 3 ? min(max(cz, 0), P.Ncells[2] - 1) : 0;
+      atomicOr(&flag[sim], 1);
+    }
+    c = T.cell_off[sim] + (cz * P.Ncells[1] + cy) * P.Ncells[0] + cx;
     cell[i] = c;
-    if (c == -1) atomicOr(flag, 1);
   }
   // Particles arrive in the previous step's cell order, so neighbouring lanes
   // mostly share a cell: one atomicAdd per run of equal cells in the wavefront,
@@ -1489,7 +1495,7 @@ class DeviceImpl final : public Device {
     HIPCHK(cell_.alloc(N_)); HIPCHK(slot_.alloc(N_)); HIPCHK(rank_.alloc(N_));
     HIPCHK(stage_src_.alloc(N_)); HIPCHK(stage_id_.alloc(N_)); HIPCHK(sorted_cell_.alloc(N_));
     HIPCHK(count_.alloc(ncells_)); HIPCHK(cs_.alloc(ncells_ + 1));
-    HIPCHK(flag_.alloc(1));
+    HIPCHK(flag_.alloc(S));
     // Neighbour-list capacity: 1.5x the mean number of particles inside the
     // cut-off sphere (disc in 2D) at the initial number density, plus 64, for
     // the densest simulation of the batch.
@@ -1585,7 +1591,7 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemset(press_.p, 0, N_ * sizeof(R)));
     HIPCHK(hipMemset(count_.p, 0, ncells_ * sizeof(int)));
     HIPCHK(hipMemset(cs_.p, 0, (ncells_ + 1) * sizeof(int)));
-    HIPCHK(hipMemset(flag_.p, 0, sizeof(int)));
+    HIPCHK(hipMemset(flag_.p, 0, S * sizeof(int)));
     HIPCHK(hipMemset(fwall_.p, 0, 6 * S * sizeof(double)));
     HIPCHK(hipMemset(steps_.p, 0, S * sizeof(int)));
     HIPCHK(hipMemset(mag_.p, 0, 3 * S * sizeof(double)));
@@ -1913,11 +1919,6 @@ class DeviceImpl final : public Device {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ke, ke_out_.p, S * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    for (int s = 0; s < S; ++s)
-      if (!std::isfinite(ke[s])) {
-        err = "kinetic energy is not finite";
-        return SPH_ERR_NAN;
-      }
     return SPH_OK;
   }
 
@@ -1927,15 +1928,29 @@ class DeviceImpl final : public Device {
   }
 
   int check_flags(std::string &err) override {
-    int flag = 0;
-    HIPCHK(hipMemcpyAsync(&flag, flag_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    std::vector<int> f;
+    int rc = poll_flags(f, err);
+    if (rc) return rc;
+    for (int v : f)
+      if (v) {
+        err = "kernel cell ranges error: a particle has not a proper cell assigned";
+        return SPH_ERR_OUT_OF_GRID;
+      }
+    return SPH_OK;
+  }
+
+  int poll_flags(std::vector<int> &flags, std::string &err) override {
+    const int S = nsim();
+    flags.assign(S, 0);
+    HIPCHK(hipMemcpyAsync(flags.data(), flag_.p, S * sizeof(int), hipMemcpyDeviceToHost,
+                          stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    if (flag) {
-      HIPCHK(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
-      HIPCHK(hipStreamSynchronize(stream_));
-      err = "kernel cell ranges error: a particle has not a proper cell assigned";
-      return SPH_ERR_OUT_OF_GRID;
-    }
+    for (int v : flags)
+      if (v) {
+        HIPCHK(hipMemsetAsync(flag_.p, 0, S * sizeof(int), stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        break;
+      }
     return SPH_OK;
   }
 

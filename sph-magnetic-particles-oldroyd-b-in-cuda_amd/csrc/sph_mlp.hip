@@ -1,0 +1,582 @@
+// sph_mlp.hip - the sweep surrogate: an ensemble of small MLPs trained and
+// evaluated with hand-written BF16 MFMA kernels (gfx950, FP32 accumulation).
+//
+// Each member maps 4 standardised inputs through 3 hidden layers of 256 SiLU
+// units to 3 outputs.  Weights live as FP32 masters (Adam state beside them)
+// and BF16 copies that feed the matrix cores; activations between layers are
+// BF16, pre-activations are kept in FP32 for the backward pass.  To keep every
+// GEMM a multiple of the 64 x 64 block tile, the input width is padded to 64
+// columns and the output width to 64 rows of W4; the padding is zero and stays
+// zero (its gradients are zero).
+//
+// One GEMM kernel serves the forward pass (Z = H W^T), the input gradients
+// (dH = dZ W) and the weight gradients (dW = dZ^T H): 256 threads compute a
+// 64 x 64 tile of C, staging 64 x 32 slices of A and B through LDS (transposed
+// on the way in where the operand is stored K-major), and each wavefront runs
+// 2 x 2 v_mfma_f32_16x16x32_bf16 tiles per slice.  Members are grid.z.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/sph.h"
+#include "../../include/sph_surrogate.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16_t;  // storage
+
+constexpr int kIn = 4, kHid = 256, kOut = 3;
+constexpr int kInPad = 64, kOutPad = 64;
+constexpr int kTile = 64, kKc = 32, kLdsStride = kKc + 8;  // +8 bf16 keeps 16-byte alignment
+
+__device__ __forceinline__ float bf2f(bf16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+// Round to nearest even (finite inputs).
+__host__ __device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u;
+#ifdef __HIP_DEVICE_COMPILE__
+  u = __float_as_uint(f);
+#else
+  std::memcpy(&u, &f, 4);
+#endif
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+enum Epi { E_FWD_SILU, E_FWD_LINEAR, E_BWD_SILU, E_STORE };
+
+// C[m][n] = sum_k A(m, k) B(k, n) for one member (blockIdx.z).
+//   A(m, k) = AT ? A[k * lda + m] : A[m * lda + k]
+//   B(k, n) = BN ? B[n * ldb + k] : B[k * ldb + n]
+// M, N multiples of 64, K a multiple of 32.  Epilogues:
+//   E_FWD_SILU   z = C + bias[n]: Z[m][n] = z, H[m][n] = bf16(silu(z))
+//   E_FWD_LINEAR Z[m][n] = C + bias[n]
+//   E_BWD_SILU   H[m][n] = bf16(C * silu'(Zp[m][n]))
+//   E_STORE      Z[m][n] = C
+struct GemmArgs {
+  const bf16_t *A, *B;
+  long sA, sB;  // member strides (elements)
+  int lda, ldb, M, N, K;
+  const float *bias;
+  long sbias;
+  float *Z;     // fp32 out [M][N]
+  bf16_t *H;    // bf16 out [M][N]
+  const float *Zp;  // pre-activation for E_BWD_SILU
+  long sZ, sH, sZp;
+};
+
+template <bool AT, bool BN, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16_t As[kTile][kLdsStride];
+  __shared__ __attribute__((aligned(16))) bf16_t Bs[kTile][kLdsStride];
+  const int e = blockIdx.z;
+  const bf16_t *A = g.A + e * g.sA, *B = g.B + e * g.sB;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < g.K; k0 += kKc) {
+    // stage A(m0.., k0..) as As[m][k] and B(k0.., n0..) as Bs[n][k]; 8 values per thread
+    if (!AT) {
+      const int r = t >> 2, c = (t & 3) * 8;
+      *(uint4 *)&As[r][c] = *(const uint4 *)&A[(long)(m0 + r) * g.lda + k0 + c];
+    } else {
+      const int kk = t >> 3, c = (t & 7) * 8;
+      uint4 v = *(const uint4 *)&A[(long)(k0 + kk) * g.lda + m0 + c];
+      const bf16_t *pv = (const bf16_t *)&v;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) As[c + q][kk] = pv[q];
+    }
+    if (BN) {
+      const int r = t >> 2, c = (t & 3) * 8;
+      *(uint4 *)&Bs[r][c] = *(const uint4 *)&B[(long)(n0 + r) * g.ldb + k0 + c];
+    } else {
+      const int kk = t >> 3, c = (t & 7) * 8;
+      uint4 v = *(const uint4 *)&B[(long)(k0 + kk) * g.ldb + n0 + c];
+      const bf16_t *pv = (const bf16_t *)&v;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Bs[c + q][kk] = pv[q];
+    }
+    __syncthreads();
+    // lane l: row (or column) l % 16 of the 16-wide tile, k = 8 (l / 16) .. + 7
+    const int rr = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      af[i] = *(const bf16x8 *)&As[wm + i * 16 + rr][kq];
+      bfr[i] = *(const bf16x8 *)&Bs[wn + i * 16 + rr][kq];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // C tile element r of lane l: row 4 (l / 16) + r, column l % 16
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn + j * 16 + (lane & 15);
+        const long o = (long)m * g.N + n;
+        const float c = acc[i][j][r];
+        if (EPI == E_FWD_SILU) {
+          const float z = c + g.bias[e * g.sbias + n];
+          g.Z[e * g.sZ + o] = z;
+          g.H[e * g.sH + o] = f2bf(silu(z));
+        } else if (EPI == E_FWD_LINEAR) {
+          g.Z[e * g.sZ + o] = c + g.bias[e * g.sbias + n];
+        } else if (EPI == E_BWD_SILU) {
+          g.H[e * g.sH + o] = f2bf(c * dsilu(g.Zp[e * g.sZp + o]));
+        } else {
+          g.Z[e * g.sZ + o] = c;
+        }
+      }
+}
+
+// Inputs [E][B][4] fp32 -> bf16 [E][B][64] (zero padding).
+__global__ void k_pack_input(const float *x, bf16_t *h, int B) {
+  const int e = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * kInPad) return;
+  const int b = i / kInPad, c = i % kInPad;
+  h[(long)e * B * kInPad + i] = c < kIn ? f2bf(x[((long)e * B + b) * kIn + c]) : (bf16_t)0;
+}
+
+// Masked mean-square loss over the first 3 outputs: per member
+// L = sum m (y - t)^2 / max(1, sum m); dY = 2 m (y - t) / max(1, sum m) as bf16
+// [E][B][64]; loss[e] = L.  One block of 256 threads per member.
+__global__ __launch_bounds__(256) void k_loss(const float *y, const float *t, const float *mask,
+                                              bf16_t *dy, float *loss, int B) {
+  const int e = blockIdx.x;
+  __shared__ float red[2][4];
+  float se = 0.f, sm = 0.f;
+  for (int i = threadIdx.x; i < B * kOut; i += 256) {
+    const int b = i / kOut, c = i % kOut;
+    const float m = mask[((long)e * B + b) * kOut + c];
+    const float d = y[((long)e * B + b) * kOutPad + c] - t[((long)e * B + b) * kOut + c];
+    se += m * d * d;
+    sm += m;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    se += __shfl_down(se, off, 64);
+    sm += __shfl_down(sm, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = se;
+    red[1][threadIdx.x >> 6] = sm;
+  }
+  __syncthreads();
+  const float SE = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+  const float SM = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  const float inv = 1.0f / fmaxf(1.0f, SM);
+  for (int i = threadIdx.x; i < B * kOutPad; i += 256) {
+    const int b = i / kOutPad, c = i % kOutPad;
+    float g = 0.f;
+    if (c < kOut) {
+      const float m = mask[((long)e * B + b) * kOut + c];
+      g = 2.0f * m * (y[((long)e * B + b) * kOutPad + c] - t[((long)e * B + b) * kOut + c]) * inv;
+    }
+    dy[(long)e * B * kOutPad + i] = f2bf(g);
+  }
+  if (threadIdx.x == 0) loss[e] = SE * inv;
+}
+
+// db[e * sdb + n] = sum_b dZ[e][b][n] (bf16 in, fp32 out), fixed order.
+__global__ void k_colsum(const bf16_t *dz, float *db, long sdb, int B, int N) {
+  const int e = blockIdx.y, n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += bf2f(dz[((long)e * B + b) * N + n]);
+  db[e * sdb + n] = s;
+}
+
+// Adam on the fp32 masters; refreshes the bf16 copies (weights only).
+__global__ void k_adam(float *w, float *m, float *v, const float *g, bf16_t *wb, long n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float *w, bf16_t *wb, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) wb[i] = f2bf(w[i]);
+}
+
+template <class T>
+struct Buf {
+  T *p = nullptr;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) e = hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
+    return e;
+  }
+};
+
+// Parameter blocks of one member, in the order of the flat parameter vector.
+struct Layout {
+  long w[4], b[4];    // offsets within a member
+  long nw[4], nb[4];  // sizes (padded)
+  int rows[4], cols[4];
+  long per_member;
+};
+
+Layout make_layout() {
+  Layout L;
+  const int rows[4] = {kHid, kHid, kHid, kOutPad}, cols[4] = {kInPad, kHid, kHid, kHid};
+  long off = 0;
+  for (int l = 0; l < 4; ++l) {
+    L.rows[l] = rows[l];
+    L.cols[l] = cols[l];
+    L.w[l] = off;
+    L.nw[l] = (long)rows[l] * cols[l];
+    off += L.nw[l];
+  }
+  for (int l = 0; l < 4; ++l) {
+    L.b[l] = off;
+    L.nb[l] = rows[l];
+    off += L.nb[l];
+  }
+  L.per_member = off;
+  return L;
+}
+
+}  // namespace
+
+struct sph_mlp {
+  int E = 0, B = 0;
+  Layout L;
+  std::string err;
+  hipStream_t stream = nullptr;
+  long step = 0;
+  // parameters: fp32 masters, Adam moments, gradients, bf16 copies - all
+  // [E][per_member]
+  Buf<float> w, m, v, g;
+  Buf<bf16_t> wb;
+  // activations
+  Buf<float> x, t, mask, z[4], loss;
+  Buf<bf16_t> h[4], dz[4];  // h[0] = packed input; h[l] = output of layer l (l = 1..3)
+  ~sph_mlp() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+#define MLPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) {                                            \
+      h->err = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+      return SPH_ERR_DEVICE;                                           \
+    }                                                                  \
+  } while (0)
+
+int alloc_batch(sph_mlp_handle h, int B) {
+  if (B == h->B) return SPH_OK;
+  const int E = h->E;
+  MLPCHK(h->x.alloc((size_t)E * B * kIn));
+  MLPCHK(h->t.alloc((size_t)E * B * kOut));
+  MLPCHK(h->mask.alloc((size_t)E * B * kOut));
+  MLPCHK(h->h[0].alloc((size_t)E * B * kInPad));
+  for (int l = 1; l < 4; ++l) MLPCHK(h->h[l].alloc((size_t)E * B * kHid));
+  for (int l = 0; l < 3; ++l) {
+    MLPCHK(h->z[l].alloc((size_t)E * B * kHid));
+    MLPCHK(h->dz[l].alloc((size_t)E * B * kHid));
+  }
+  MLPCHK(h->z[3].alloc((size_t)E * B * kOutPad));
+  MLPCHK(h->dz[3].alloc((size_t)E * B * kOutPad));
+  MLPCHK(h->loss.alloc(E));
+  h->B = B;
+  return SPH_OK;
+}
+
+template <bool AT, bool BN, int EPI>
+int gemm(sph_mlp_handle h, const GemmArgs &a) {
+  if (a.M % kTile || a.N % kTile || a.K % kKc) {
+    h->err = "GEMM shape not a multiple of the tile";
+    return SPH_ERR_ARG;
+  }
+  dim3 grid(a.N / kTile, a.M / kTile, h->E);
+  k_gemm<AT, BN, EPI><<<grid, 256, 0, h->stream>>>(a);
+  MLPCHK(hipGetLastError());
+  return SPH_OK;
+}
+
+// Forward of every member on the current batch: h[0] -> z[3] (outputs in the
+// first 3 columns of z[3]).
+int forward(sph_mlp_handle h) {
+  const int E = h->E, B = h->B;
+  const Layout &L = h->L;
+  k_pack_input<<<dim3((B * kInPad + 255) / 256, E), 256, 0, h->stream>>>(h->x.p, h->h[0].p, B);
+  MLPCHK(hipGetLastError());
+  for (int l = 0; l < 4; ++l) {
+    GemmArgs a{};
+    a.A = h->h[l].p;
+    a.sA = (long)B * L.cols[l];
+    a.lda = L.cols[l];
+    a.B = h->wb.p + L.w[l];
+    a.sB = L.per_member;
+    a.ldb = L.cols[l];
+    a.M = B;
+    a.N = L.rows[l];
+    a.K = L.cols[l];
+    a.bias = h->w.p + L.b[l];
+    a.sbias = L.per_member;
+    a.Z = h->z[l].p;
+    a.sZ = (long)B * L.rows[l];
+    int rc;
+    if (l < 3) {
+      a.H = h->h[l + 1].p;
+      a.sH = (long)B * L.rows[l];
+      rc = gemm<false, true, E_FWD_SILU>(h, a);
+    } else {
+      rc = gemm<false, true, E_FWD_LINEAR>(h, a);
+    }
+    if (rc) return rc;
+  }
+  return SPH_OK;
+}
+
+// Loss and gradients of every member on the current batch into h->g.
+int backward(sph_mlp_handle h) {
+  const int E = h->E, B = h->B;
+  const Layout &L = h->L;
+  k_loss<<<E, 256, 0, h->stream>>>(h->z[3].p, h->t.p, h->mask.p, h->dz[3].p, h->loss.p, B);
+  MLPCHK(hipGetLastError());
+  for (int l = 3; l >= 0; --l) {
+    // dW_l = dZ_l^T H_l   ([rows x B] x [B x cols])
+    GemmArgs a{};
+    a.A = h->dz[l].p;
+    a.sA = (long)B * L.rows[l];
+    a.lda = L.rows[l];
+    a.B = h->h[l].p;
+    a.sB = (long)B * L.cols[l];
+    a.ldb = L.cols[l];
+    a.M = L.rows[l];
+    a.N = L.cols[l];
+    a.K = B;
+    a.Z = h->g.p + L.w[l];
+    a.sZ = L.per_member;
+    int rc = gemm<true, false, E_STORE>(h, a);
+    if (rc) return rc;
+    k_colsum<<<dim3((L.rows[l] + 255) / 256, E), 256, 0, h->stream>>>(
+        h->dz[l].p, h->g.p + L.b[l], L.per_member, B, L.rows[l]);
+    MLPCHK(hipGetLastError());
+    if (l == 0) break;
+    // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1})   ([B x rows] x [rows x cols])
+    GemmArgs d{};
+    d.A = h->dz[l].p;
+    d.sA = (long)B * L.rows[l];
+    d.lda = L.rows[l];
+    d.B = h->wb.p + L.w[l];
+    d.sB = L.per_member;
+    d.ldb = L.cols[l];
+    d.M = B;
+    d.N = L.cols[l];
+    d.K = L.rows[l];
+    d.H = h->dz[l - 1].p;
+    d.sH = (long)B * L.cols[l];
+    d.Zp = h->z[l - 1].p;
+    d.sZp = (long)B * L.cols[l];
+    if ((rc = gemm<false, false, E_BWD_SILU>(h, d))) return rc;
+  }
+  return SPH_OK;
+}
+
+int put(sph_mlp_handle h, float *dst, const float *src, size_t n) {
+  MLPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  return SPH_OK;
+}
+
+int load_batch(sph_mlp_handle h, int B, const float *x, const float *t, const float *mask) {
+  if (B <= 0 || B % kTile) {
+    h->err = "the batch must be a positive multiple of 64";
+    return SPH_ERR_ARG;
+  }
+  int rc = alloc_batch(h, B);
+  if (rc) return rc;
+  const size_t E = h->E;
+  if ((rc = put(h, h->x.p, x, E * B * kIn))) return rc;
+  if (t && (rc = put(h, h->t.p, t, E * B * kOut))) return rc;
+  if (mask && (rc = put(h, h->mask.p, mask, E * B * kOut))) return rc;
+  return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sph_mlp_handle sph_mlp_create(int members, uint64_t seed) {
+  if (members < 1 || members > 64) return nullptr;
+  sph_mlp_handle h = new sph_mlp();
+  h->E = members;
+  h->L = make_layout();
+  const size_t n = (size_t)members * h->L.per_member;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      h->w.alloc(n) != hipSuccess || h->m.alloc(n) != hipSuccess ||
+      h->v.alloc(n) != hipSuccess || h->g.alloc(n) != hipSuccess ||
+      h->wb.alloc(n) != hipSuccess) {
+    delete h;
+    return nullptr;
+  }
+  // He-uniform weights (bound sqrt(6 / fan_in)), zero biases, per member.
+  std::mt19937_64 rng(seed);
+  std::vector<float> w(n, 0.0f);
+  const Layout &L = h->L;
+  const int fan_in[4] = {kIn, kHid, kHid, kHid}, real_rows[4] = {kHid, kHid, kHid, kOut},
+            real_cols[4] = {kIn, kHid, kHid, kHid};
+  for (int e = 0; e < members; ++e)
+    for (int l = 0; l < 4; ++l) {
+      const float bound = std::sqrt(6.0f / fan_in[l]) * (l == 3 ? 0.1f : 1.0f);
+      std::uniform_real_distribution<float> U(-bound, bound);
+      for (int r = 0; r < real_rows[l]; ++r)
+        for (int c = 0; c < real_cols[l]; ++c)
+          w[(size_t)e * L.per_member + L.w[l] + (size_t)r * L.cols[l] + c] = U(rng);
+    }
+  if (sph_mlp_set_params(h, w.data(), (int64_t)n) != SPH_OK) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void sph_mlp_destroy(sph_mlp_handle h) { delete h; }
+
+const char *sph_mlp_last_error(sph_mlp_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int64_t sph_mlp_param_count(sph_mlp_handle h) {
+  return h ? (int64_t)h->E * h->L.per_member : -1;
+}
+
+int sph_mlp_layout(sph_mlp_handle h, int64_t offsets[8], int32_t shapes[8]) {
+  if (!h || !offsets || !shapes) return SPH_ERR_ARG;
+  for (int l = 0; l < 4; ++l) {
+    offsets[l] = h->L.w[l];
+    offsets[4 + l] = h->L.b[l];
+    shapes[2 * l] = h->L.rows[l];
+    shapes[2 * l + 1] = h->L.cols[l];
+  }
+  return SPH_OK;
+}
+
+int sph_mlp_set_params(sph_mlp_handle h, const float *params, int64_t count) {
+  if (!h || !params) return SPH_ERR_ARG;
+  const long n = (long)h->E * h->L.per_member;
+  if (count != n) {
+    h->err = "parameter count mismatch";
+    return SPH_ERR_ARG;
+  }
+  MLPCHK(hipMemcpy(h->w.p, params, n * sizeof(float), hipMemcpyHostToDevice));
+  MLPCHK(hipMemset(h->m.p, 0, n * sizeof(float)));
+  MLPCHK(hipMemset(h->v.p, 0, n * sizeof(float)));
+  k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->w.p, h->wb.p, n);
+  MLPCHK(hipGetLastError());
+  MLPCHK(hipStreamSynchronize(h->stream));
+  h->step = 0;
+  return SPH_OK;
+}
+
+int sph_mlp_get_params(sph_mlp_handle h, float *params, int64_t count) {
+  if (!h || !params) return SPH_ERR_ARG;
+  const long n = (long)h->E * h->L.per_member;
+  if (count != n) {
+    h->err = "parameter count mismatch";
+    return SPH_ERR_ARG;
+  }
+  MLPCHK(hipStreamSynchronize(h->stream));
+  MLPCHK(hipMemcpy(params, h->w.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return SPH_OK;
+}
+
+int sph_mlp_forward(sph_mlp_handle h, int batch, const float *x, float *y) {
+  if (!h || !x || !y) return SPH_ERR_ARG;
+  int rc = load_batch(h, batch, x, nullptr, nullptr);
+  if (rc || (rc = forward(h))) return rc;
+  // first 3 of the 64 output columns
+  std::vector<float> tmp((size_t)h->E * batch * kOutPad);
+  MLPCHK(hipMemcpyAsync(tmp.data(), h->z[3].p, tmp.size() * sizeof(float),
+                        hipMemcpyDeviceToHost, h->stream));
+  MLPCHK(hipStreamSynchronize(h->stream));
+  for (size_t r = 0; r < (size_t)h->E * batch; ++r)
+    for (int c = 0; c < kOut; ++c) y[r * kOut + c] = tmp[r * kOutPad + c];
+  return SPH_OK;
+}
+
+int sph_mlp_gradients(sph_mlp_handle h, int batch, const float *x, const float *t,
+                      const float *mask, float *grads, float *loss) {
+  if (!h || !x || !t || !mask) return SPH_ERR_ARG;
+  int rc = load_batch(h, batch, x, t, mask);
+  if (rc || (rc = forward(h)) || (rc = backward(h))) return rc;
+  const long n = (long)h->E * h->L.per_member;
+  if (grads)
+    MLPCHK(hipMemcpyAsync(grads, h->g.p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (loss)
+    MLPCHK(hipMemcpyAsync(loss, h->loss.p, h->E * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+  MLPCHK(hipStreamSynchronize(h->stream));
+  return SPH_OK;
+}
+
+int sph_mlp_train_step(sph_mlp_handle h, int batch, const float *x, const float *t,
+                       const float *mask, float lr, float *loss) {
+  if (!h || !x || !t || !mask) return SPH_ERR_ARG;
+  int rc = load_batch(h, batch, x, t, mask);
+  if (rc || (rc = forward(h)) || (rc = backward(h))) return rc;
+  const long n = (long)h->E * h->L.per_member;
+  h->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f - std::pow(b1, (float)h->step), c2 = 1.0f - std::pow(b2, (float)h->step);
+  k_adam<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->w.p, h->m.p, h->v.p, h->g.p,
+                                                             h->wb.p, n, lr, b1, b2, eps, c1, c2);
+  MLPCHK(hipGetLastError());
+  if (loss) {
+    MLPCHK(hipMemcpyAsync(loss, h->loss.p, h->E * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+    MLPCHK(hipStreamSynchronize(h->stream));
+  }
+  return SPH_OK;
+}
+
+int sph_mlp_synchronize(sph_mlp_handle h) {
+  if (!h) return SPH_ERR_ARG;
+  MLPCHK(hipStreamSynchronize(h->stream));
+  return SPH_OK;
+}
+
+}  // extern "C"

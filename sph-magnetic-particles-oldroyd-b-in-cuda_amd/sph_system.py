@@ -40,7 +40,13 @@ ENTRY_POINTS = [
     "sph_batch_size", "sph_batch_prepare", "sph_batch_step", "sph_batch_synchronize",
     "sph_batch_current_step", "sph_batch_get_field", "sph_batch_kinetic_energy",
     "sph_batch_observables", "sph_batch_set_launch", "sph_batch_get_launch",
-    "sph_batch_autotune",
+    "sph_batch_autotune", "sph_batch_failed",
+]
+# include/sph_surrogate.h (bound in surrogate.py)
+SURROGATE_ENTRY_POINTS = [
+    "sph_mlp_create", "sph_mlp_destroy", "sph_mlp_last_error", "sph_mlp_param_count",
+    "sph_mlp_layout", "sph_mlp_set_params", "sph_mlp_get_params", "sph_mlp_forward",
+    "sph_mlp_gradients", "sph_mlp_train_step", "sph_mlp_synchronize",
 ]
 OBSERVABLES = ["kinetic_energy", "polymer_N1", "polymer_shear", "wall_shear_bottom",
                "wall_shear_top", "chain_angle", "field_angle", "phase_lag"]
@@ -175,7 +181,7 @@ def library(path=None):
     lib.sph_batch_step.argtypes = [vp, i32]
     lib.sph_batch_get_field.restype = i32
     lib.sph_batch_get_field.argtypes = [vp, i32, cp, vp, i64]
-    for fn in ("sph_batch_kinetic_energy", "sph_batch_observables"):
+    for fn in ("sph_batch_kinetic_energy", "sph_batch_observables", "sph_batch_failed"):
         getattr(lib, fn).restype = i32
         getattr(lib, fn).argtypes = [vp, vp]
     lib.sph_batch_set_launch.restype = i32
@@ -454,6 +460,13 @@ class Batch:
     def kinetic_energy(self):
         out = np.zeros(len(self), np.float64)
         self._check(self.lib.sph_batch_kinetic_energy(self.h, _ptr(out)), "batch_kinetic_energy")
+        return out
+
+    def failed(self):
+        """Per simulation: 0, or bit 1 (a particle left the grid) | bit 2
+        (non-finite kinetic energy); sticky."""
+        out = np.zeros(len(self), np.int32)
+        self._check(self.lib.sph_batch_failed(self.h, _ptr(out)), "batch_failed")
         return out
 
     def observables(self):

@@ -10,19 +10,20 @@ import sph_system
 from conftest import ROOT
 
 
-def header_functions():
-    text = open(os.path.join(ROOT, "include", "sph.h")).read()
+def header_functions(name="sph.h"):
+    text = open(os.path.join(ROOT, "include", name)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(sph_[A-Za-z0-9_]+)\s*\(", text)))
 
 
 def test_header_matches_entry_point_list():
     assert header_functions() == sorted(sph_system.ENTRY_POINTS)
+    assert header_functions("sph_surrogate.h") == sorted(sph_system.SURROGATE_ENTRY_POINTS)
 
 
 def test_library_exports_every_header_symbol():
     lib = ctypes.CDLL(sph_system.LIB_PATH)
-    for name in header_functions():
+    for name in header_functions() + header_functions("sph_surrogate.h"):
         assert hasattr(lib, name), name
 
 

@@ -142,3 +142,26 @@ def test_batch_of_one_equals_system(tmp_path):
     b.step(5)
     s.step(5)
     _compare(b, [s], "after 5 steps")
+
+
+def test_failed_simulation_is_flagged_and_isolated(tmp_path):
+    """A particle moved beyond a wall band in one simulation flags that
+    simulation only; the others reproduce their stand-alone runs bit for bit."""
+    singles = _systems("2d", str(tmp_path), sph_system.FP64, "s")
+    members = _systems("2d", str(tmp_path), sph_system.FP64, "b")
+    y = members[1].get("y")
+    y[5] = 3.0
+    members[1].set("y", y)
+    b = sph_system.Batch(sph_system.FP64)
+    for s in members:
+        b.add(s)
+    b.prepare()
+    b.step(3)
+    flags = b.failed()
+    assert flags[1] & 1 and not flags[0] and not flags[2] and not flags[3]
+    for k in (0, 2, 3):
+        s = singles[k]
+        s.prepare()
+        s.step(3)
+        for n in _names(s.params):
+            np.testing.assert_array_equal(b.get(k, n), s.get(n), err_msg=f"sim {k} {n}")

@@ -1,0 +1,50 @@
+"""Sweeps run as batches, and what the surrogate gains over plain batched SPH."""
+import time
+
+import numpy as np
+import pytest
+
+import surrogate
+import sweep
+
+pytestmark = pytest.mark.gpu
+
+
+def test_grouping_into_batches_does_not_change_results():
+    pts = sweep.resolve(sweep.sample_points(12, seed=7, n_range=(32, 48)))
+    a, ia = sweep.run_points([dict(p) for p in pts], 200, max_particles=10**9)
+    b, ib = sweep.run_points([dict(p) for p in pts], 200, max_particles=8000)
+    assert ib["batches"] > ia["batches"] == 1
+    np.testing.assert_array_equal(a, b)
+    assert np.all(np.isfinite(a[:, :2]))
+    # walls moving apart shear the fluid: positive wall stress, growing with V
+    V = np.array([p["V"] for p in pts])
+    assert np.all(a[V > 0.05, 0] > 0)
+    assert np.corrcoef(V, a[:, 0])[0, 1] > 0.9
+
+
+def test_surrogate_sweep_gains_over_plain_batched_sph():
+    """Train on 600 SPH runs, answer 60 new points: every point whose ensemble
+    spread of the wall stress exceeds 5 % is re-run with SPH.  Most points are
+    answered by the network, the pipeline is faster than running all 60 with
+    batched SPH, and its wall stress is within 5 % (median relative error) of
+    the SPH answer."""
+    steps = 400
+    train = sweep.resolve(sweep.sample_points(600, seed=11, n_range=(32, 40)))
+    Ytr, _ = sweep.run_points([dict(p) for p in train], steps)
+    model = surrogate.Surrogate(members=5, seed=0)
+    hist = model.fit(surrogate.feature_matrix(train), Ytr, steps=1500, batch=256, lr=1e-3)
+    assert hist[-50:].mean() < 0.2 * hist[:50].mean()
+    pts = sweep.resolve(sweep.sample_points(60, seed=12, n_range=(32, 40)))
+    t0 = time.perf_counter()
+    Yref, _ = sweep.run_points([dict(p) for p in pts], steps)
+    t_plain = time.perf_counter() - t0
+    Ys, info = surrogate.sweep_with_fallback(pts, model, steps, threshold=0.05)
+    ok = np.isfinite(Yref[:, 0])                     # points whose SPH run succeeded
+    assert ok.mean() > 0.9
+    err = np.abs(Ys[ok, 0] - Yref[ok, 0]) / np.maximum(np.abs(Yref[ok, 0]), 1e-12)
+    print(f"plain {t_plain:.3f} s, surrogate {info['seconds']:.3f} s "
+          f"({info['fallback']} of {len(pts)} re-run), median err {np.median(err):.3e}")
+    assert info["fallback"] < 0.5 * len(pts)
+    assert info["seconds"] < t_plain
+    assert np.median(err) <= 0.05
