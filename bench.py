@@ -24,6 +24,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "sph-magnetic-particles-oldroyd-b-in-cuda_amd")
 ORACLE = os.path.join(ROOT, "oracle")
@@ -31,6 +33,7 @@ sys.path[:0] = [PKG, ORACLE]
 
 import inputs  # noqa: E402
 import sph_system  # noqa: E402
+import surrogate  # noqa: E402
 import sweep  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -144,14 +147,18 @@ def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
     b.close()
     # one simulation at a time: same kernels, one launch set per simulation
     t_seq = 0.0
+    failed = 0
     for s in systems:
-        s.prepare()
-        s.step(2)
-        s.synchronize()
-        t0 = time.perf_counter()
-        s.step(seq_steps)
-        s.synchronize()
-        t_seq += time.perf_counter() - t0
+        try:
+            s.prepare()
+            s.step(2)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(seq_steps)
+            s.synchronize()
+            t_seq += time.perf_counter() - t0
+        except sph_system.SphError:
+            failed += 1
         s.close()
     batched = n_part * steps / t_batch
     sequential = n_part * seq_steps / t_seq
@@ -161,8 +168,46 @@ def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
         "value": batched, "unit": "particle-steps/s",
         "ms_per_step": 1e3 * t_batch / steps,
         "launch": {"block": block, "xcd": xcd},
-        "sequential": sequential, "sequential_steps": seq_steps,
+        "sequential": sequential, "sequential_steps": seq_steps, "sequential_failed": failed,
         "batch_speedup": batched / sequential,
+    }
+
+
+def time_pipeline(n_train, n_hold, n_sweep, steps, n=48, seed=21, log=None):
+    """The learned sweep: SPH runs train the MLP ensemble; a new sweep is then
+    answered by the ensemble, with the points whose spread of the wall stress
+    exceeds 5 % re-run by batched SPH.  Compared with running the whole new
+    sweep with batched SPH: wall time and wall-stress error.  The lattice is
+    fixed (n x n) so that the outputs are functions of the four inputs; the
+    physical parameters are drawn as in the uneven-size sweep."""
+    train = sweep.resolve(sweep.sample_points(n_train + n_hold, seed=seed, n_range=(n, n)))
+    Y, tinfo = sweep.run_points(train, steps, log=log)
+    X = surrogate.feature_matrix(train)
+    model = surrogate.Surrogate(members=5, seed=seed)
+    t0 = time.perf_counter()
+    model.fit(X[:n_train], Y[:n_train], steps=6000, batch=256, lr=2e-3, lr_end=1e-5)
+    t_fit = time.perf_counter() - t0
+    mh, _ = model.predict(X[n_train:])
+    ok = np.isfinite(Y[n_train:, 0])
+    held_err = np.abs(mh[ok, 0] - Y[n_train:][ok, 0]) / np.abs(Y[n_train:][ok, 0])
+    pts = sweep.resolve(sweep.sample_points(n_sweep, seed=seed + 1, n_range=(n, n)))
+    t0 = time.perf_counter()
+    Yref, pinfo = sweep.run_points([dict(p) for p in pts], steps, log=log)
+    t_plain = time.perf_counter() - t0
+    Ys, info = surrogate.sweep_with_fallback(pts, model, steps, threshold=0.05, log=log)
+    ok = np.isfinite(Yref[:, 0]) & np.isfinite(Ys[:, 0])
+    err = np.abs(Ys[ok, 0] - Yref[ok, 0]) / np.abs(Yref[ok, 0])
+    return {
+        "workload": f"{n_sweep}-point 2D sweep ({n}x{n} lattice, {steps} steps, dt {sweep.DT}); "
+                    f"surrogate trained on {n_train} SPH runs, {n_hold} held out",
+        "plain_batched_s": t_plain, "surrogate_s": info["seconds"],
+        "speedup": t_plain / info["seconds"],
+        "fallback_points": info["fallback"], "inference_s": info["inference_s"],
+        "wall_stress_rel_err_median": float(np.median(err)),
+        "wall_stress_rel_err_p90": float(np.percentile(err, 90)),
+        "held_out_rel_err_median": float(np.median(held_err)),
+        "failed_sph_runs": int(pinfo["failed"] + tinfo["failed"]),
+        "train_data_s": tinfo["seconds"], "fit_s": t_fit,
     }
 
 
@@ -179,6 +224,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-sims", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=200)
+    ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--pipeline-points", type=int, default=1000)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
@@ -199,6 +246,12 @@ def main():
             print(f"# sweep: {sw['sims']} sims, {sw['particles']} particles, "
                   f"{sw['ms_per_step']:.3f} ms/step batched, x{sw['batch_speedup']:.1f} vs one "
                   f"at a time", file=sys.stderr, flush=True)
+        pipe = None
+        if not a.no_pipeline:
+            pipe = time_pipeline(1800, 200, a.pipeline_points, 800)
+            print(f"# pipeline: x{pipe['speedup']:.1f} vs plain batched, "
+                  f"{pipe['fallback_points']} of {a.pipeline_points} re-run, median wall-stress "
+                  f"error {pipe['wall_stress_rel_err_median']:.3%}", file=sys.stderr, flush=True)
         cpu = None
         if not a.no_cpu:
             cpu = cpu_baseline(a.n, a.colloids, a.cpu_steps, tmpdir)
@@ -218,7 +271,7 @@ def main():
     traffic = measured_traffic(kname, N)
     step_ms = 1e3 * r32["elapsed"] / a.steps
     line = {
-        "metric": "particle-steps/s, 3D 1M-particle Oldroyd-B (FP32 vs FP64)",
+        "metric": "particle-steps/s, 3D 1M-particle Oldroyd-B (FP32 vs FP64); % HBM peak; sweep speedup",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -257,6 +310,7 @@ def main():
             "avg_launch_ms": avg_ms,
         },
         "sweep": sw,
+        "sweep_pipeline": pipe,
         "cpu_baseline": None if cpu is None else {
             "value": cpu["value"],
             "unit": "particle-steps/s",

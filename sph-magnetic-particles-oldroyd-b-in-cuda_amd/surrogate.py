@@ -151,18 +151,32 @@ def feature_matrix(points):
 
 
 class Surrogate:
-    """Ensemble + standardisation + bootstrap training + spread-based fallback."""
+    """Ensemble + standardisation + bootstrap training + spread-based fallback.
+
+    The network learns the wall stress as the apparent viscosity
+    sigma / (eta gamma_dot) = sigma / (2 V) (eta = 1, L1 = 1), which stays O(1)
+    for every wall speed, and multiplies back by 2 V; its spread relative to its
+    mean is then the relative spread of the wall stress itself.  Rows with
+    V < 1e-3 leave that target out (the ratio is noise there)."""
 
     def __init__(self, members=5, seed=0):
         self.net = Ensemble(members, seed)
         self.seed = seed
         self.xm = self.xs = self.ym = self.ys = None
 
-    def fit(self, X, Y, steps=3000, batch=256, lr=1e-3, log_every=0, log=None):
+    @staticmethod
+    def _to_net(X, Y):
+        Y = np.array(Y, np.float64)
+        V = np.asarray(X, np.float64)[:, 3]
+        Y[:, 0] = np.where(V >= 1e-3, Y[:, 0] / (2.0 * np.maximum(V, 1e-3)), np.nan)
+        return Y
+
+    def fit(self, X, Y, steps=3000, batch=256, lr=1e-3, lr_end=None, log_every=0, log=None):
         """X [n, 4] raw features, Y [n, 3] targets (NaN = missing).  Each member
-        trains on its own bootstrap resample of the rows."""
+        trains on its own bootstrap resample of the rows.  With `lr_end` the
+        learning rate decays from `lr` to `lr_end` on a half cosine."""
         X = np.asarray(X, np.float64)
-        Y = np.asarray(Y, np.float64)
+        Y = self._to_net(X, Y)
         self.xm, self.xs = X.mean(0), X.std(0) + 1e-12
         self.ym = np.nanmean(Y, 0)
         self.ys = np.nanstd(Y, 0) + 1e-12
@@ -176,8 +190,10 @@ class Surrogate:
         boot = [rng.integers(0, n, n) for _ in range(E)]
         hist = []
         for s in range(steps):
+            rate = lr if lr_end is None else (
+                lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * s / steps)))
             idx = np.stack([b[rng.integers(0, n, batch)] for b in boot])
-            loss = self.net.train_step(Xs[idx], Ys[idx], M[idx], lr)
+            loss = self.net.train_step(Xs[idx], Ys[idx], M[idx], rate)
             hist.append(float(loss.mean()))
             if log and log_every and (s % log_every == 0 or s == steps - 1):
                 log(f"step {s}: loss {hist[-1]:.4e}")
@@ -185,8 +201,10 @@ class Surrogate:
 
     def predict(self, X):
         """(mean [n, 3], std [n, 3]) over the members, in target units."""
-        Xs = (np.asarray(X, np.float64) - self.xm) / self.xs
+        X = np.asarray(X, np.float64)
+        Xs = (X - self.xm) / self.xs
         y = self.net.forward(Xs).astype(np.float64) * self.ys + self.ym   # [E, n, 3]
+        y[:, :, 0] *= 2.0 * X[:, 3]
         return y.mean(0), y.std(0)
 
 

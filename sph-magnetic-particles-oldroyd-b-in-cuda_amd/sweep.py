@@ -55,7 +55,7 @@ def sample_points(n_points, seed, n_range=(32, 160), v_range=(0.0, 1.0),
     return pts
 
 
-DT = 3.2e-5
+DT = 1.6e-5
 # Colloid centres start at least 4 R = 0.24 from a wall: the reference has no
 # colloid-wall repulsion, and a rotating chain that reaches a wall pushes its
 # shell particles through the wall band (the run then fails).
@@ -64,9 +64,10 @@ WALL_GAP = 4.0
 
 def point_cfg(pt, steps, dt=DT):
     """Input of one sweep point.  Every point uses the same dt, so a run of
-    `steps` steps covers the same physical time whatever its resolution; the
-    default 3.2e-5 keeps n = 160 (the finest lattice) inside the sound-speed
-    and viscous limits with c = 20."""
+    `steps` steps covers the same physical time whatever its resolution.  The
+    default 1.6e-5 gives the finest lattice (n = 160) the viscous number
+    nu dt / dx^2 = 0.41 of the 64 x 64 Couette flow at dt = 1e-4 (twice that,
+    dt = 3.2e-5, blows up within 10 steps at n = 160)."""
     n = pt["n"]
     cfg = inputs.couette_2d(n=n, steps=steps, dt=dt, tau=pt["tau"],
                             Vwall_bottom=-pt["V"], Vwall_top=pt["V"])
@@ -129,6 +130,44 @@ def _circ_half(angles):
     return 0.5 * math.atan2(np.sin(2 * a).mean(), np.cos(2 * a).mean())
 
 
+def _run_batch(b, g, steps, samples, autotune, targets, info, log):
+    n_part = sum(p.N for p in b.params)
+    t0 = time.perf_counter()
+    b.prepare()
+    if autotune:
+        b.autotune(steps=2)
+    done = b.current_step()
+    first = max(done, steps // 2)
+    if first > done:
+        b.step(first - done)
+    every = max(1, (steps - first) // samples)
+    obs = []
+    k = first
+    while k < steps:
+        m = min(every, steps - k)
+        b.step(m)
+        k += m
+        obs.append(b.observables())
+    b.synchronize()
+    failed = b.failed()
+    dt = time.perf_counter() - t0
+    info["batches"] += 1
+    info["seconds"] += dt
+    info["particle_steps"] += n_part * steps
+    info["particles"].append(n_part)
+    o = np.stack(obs)                  # [samples, sims, 8]
+    for j, i in enumerate(g):
+        if failed[j]:
+            info["failed"] += 1
+            continue
+        wall = 0.5 * (o[:, j, 3] - o[:, j, 4])
+        targets[i, 0] = float(np.mean(wall))
+        targets[i, 1] = float(np.mean(o[:, j, 1]))
+        targets[i, 2] = _circ_half(o[:, j, 7])
+    if log:
+        log(f"batch of {len(g)} sims, {n_part} particles, {steps} steps: {dt:.2f} s")
+
+
 def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000,
                samples=20, autotune=False, log=None):
     """Runs every point for `steps` steps in batches of at most `max_particles`
@@ -136,63 +175,27 @@ def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000
     `samples` times over the second half of the run and averaged.  A point
     whose simulation fails (a particle leaves the grid - e.g. a colloid driven
     into a wall - or its energy is not finite) gets NaN targets and is counted
-    in info["failed"]; the rest of its batch is unaffected."""
+    in info["failed"]; the rest of its batch is unaffected.  info["seconds"]
+    is the device time of the batches (upload, steps, observables), without
+    the host set-up of the inputs."""
     targets = np.full((len(points), 3), np.nan)
     info = dict(batches=0, particle_steps=0, seconds=0.0, particles=[], failed=0)
     order = sorted(range(len(points)), key=lambda i: -points[i]["n"])
     with tempfile.TemporaryDirectory() as tmpdir:
-        group, total = [], 0
-        groups = []
-        systems = {}
+        b, g, total = None, [], 0
         for i in order:
             s = build_system(points[i], steps, precision, tmpdir)
-            if group and total + s.N > max_particles:
-                groups.append(group)
-                group, total = [], 0
-            group.append(i)
+            if g and total + s.N > max_particles:
+                _run_batch(b, g, steps, samples, autotune, targets, info, log)
+                b.close()
+                b, g, total = None, [], 0
+            if b is None:
+                b = sph_system.Batch(precision)
+            b.add(s)
+            g.append(i)
             total += s.N
-            systems[i] = s
-        if group:
-            groups.append(group)
-        for g in groups:
-            b = sph_system.Batch(precision)
-            for i in g:
-                b.add(systems[i])
-                systems[i].close()
-            n_part = sum(p.N for p in b.params)
-            t0 = time.perf_counter()
-            b.prepare()
-            if autotune:
-                b.autotune(steps=2)
-            done = b.current_step()
-            first = max(done, steps // 2)
-            if first > done:
-                b.step(first - done)
-            every = max(1, (steps - first) // samples)
-            obs = []
-            k = first
-            while k < steps:
-                m = min(every, steps - k)
-                b.step(m)
-                k += m
-                obs.append(b.observables())
-            b.synchronize()
-            failed = b.failed()
-            dt = time.perf_counter() - t0
-            info["batches"] += 1
-            info["seconds"] += dt
-            info["particle_steps"] += n_part * steps
-            info["particles"].append(n_part)
-            o = np.stack(obs)                  # [samples, sims, 8]
-            for j, i in enumerate(g):
-                if failed[j]:
-                    info["failed"] += 1
-                    continue
-                wall = 0.5 * (o[:, j, 3] - o[:, j, 4])
-                targets[i, 0] = float(np.mean(wall))
-                targets[i, 1] = float(np.mean(o[:, j, 1]))
-                targets[i, 2] = _circ_half(o[:, j, 7])
+            s.close()
+        if g:
+            _run_batch(b, g, steps, samples, autotune, targets, info, log)
             b.close()
-            if log:
-                log(f"batch of {len(g)} sims, {n_part} particles, {steps} steps: {dt:.2f} s")
     return targets, info

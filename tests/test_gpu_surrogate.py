@@ -100,13 +100,15 @@ def test_training_fits_a_smooth_map():
     more than tenfold, and its spread is small where data is dense."""
     rng = np.random.default_rng(0)
     X = rng.uniform(-1, 1, size=(2000, 4))
-    Y = np.stack([np.sin(2 * X[:, 0]) + X[:, 1] ** 2, X[:, 2] * X[:, 3] + 0.5 * X[:, 0],
-                  np.tanh(X[:, 0] + X[:, 3])], 1)
+    X[:, 3] = rng.uniform(0.5, 1.0, 2000)      # the wall-speed column (first target / 2V)
+    Y = np.stack([(np.sin(2 * X[:, 0]) + X[:, 1] ** 2 + 2.0) * 2 * X[:, 3],
+                  X[:, 2] * X[:, 3] + 0.5 * X[:, 0], np.tanh(X[:, 0] + X[:, 3])], 1)
     model = surrogate.Surrogate(members=5, seed=1)
     Xt, Yt = X[:1800], Y[:1800]
     Xh, Yh = X[1800:], Y[1800:]
     model.xm, model.xs = Xt.mean(0), Xt.std(0)
-    model.ym, model.ys = Yt.mean(0), Yt.std(0)
+    Yn = model._to_net(Xt, Yt)
+    model.ym, model.ys = Yn.mean(0), Yn.std(0)
     before = np.abs(model.predict(Xh)[0] - Yh).mean()
     hist = model.fit(Xt, Yt, steps=600, batch=256, lr=2e-3)
     mean, std = model.predict(Xh)

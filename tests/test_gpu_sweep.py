@@ -29,7 +29,7 @@ def test_surrogate_sweep_gains_over_plain_batched_sph():
     answered by the network, the pipeline is faster than running all 60 with
     batched SPH, and its wall stress is within 5 % (median relative error) of
     the SPH answer."""
-    steps = 400
+    steps = 800
     train = sweep.resolve(sweep.sample_points(600, seed=11, n_range=(32, 40)))
     Ytr, _ = sweep.run_points([dict(p) for p in train], steps)
     model = surrogate.Surrogate(members=5, seed=0)
