@@ -12,7 +12,7 @@
 // pair passes read neighbours coalesced instead of through particle_index.
 //
 // Kernels and the reference kernels they replace:
-//   k_hash, k_scan_cells, k_scatter, k_rank_gather   kernel_cell_list.cu:15-63,
+//   k_hash, k_scan_*, k_scatter, k_rank_gather   kernel_cell_list.cu:15-63,
 //       thrust::sort_by_key (system_neighbours_search.cu:56-58), kernel_cell_ranges.cu:13-33
 //   k_density            kernel_densities.cu:16-193 + kernel_pressures.cu:13-23
 //   k_gradv              kernel_vel_gradients.cu:15-662 + kernel_dcdt.cu:15-104
@@ -465,55 +465,82 @@ __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__res
   if (c >= 0) slot[i] = base + (lane - leader);
 }
 
-// Exclusive scan of the per-cell counts into cs[0..ncells]; zeroes the counts
-// for the next hash.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_cells(int *count, int *cs, int ncells) {
-  // Tiles of 8192 cells: each thread owns 8 consecutive cells of the tile, the
-  // block scans the per-thread sums, and a running carry links the tiles.
+// Exclusive scan of the per-cell counts into cs[0..ncells] in three launches:
+// each workgroup scans one tile of kScanTile cells (zeroing the counts for the
+// next hash), one workgroup scans the tile totals, and a last pass adds each
+// tile's offset.  Every workgroup has 1024 threads owning 8 consecutive values.
+constexpr int kScanTile = 8192;
+
+// Exclusive scan of v[0..8) per thread across the workgroup; returns this
+// thread's exclusive prefix and the workgroup total in *total.
+__device__ __forceinline__ int block_scan8(const int (&v)[8], int *total) {
   __shared__ int wsum[16];
-  __shared__ int tile_total;
+  __shared__ int tot;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  int carry = 0;
-  for (int base = 0; base < ncells; base += 8192) {
-    const int c0 = base + t * 8;
-    int v[8];
-    int local = 0;
+  int local = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      v[k] = c0 + k < ncells ? count[c0 + k] : 0;
-      local += v[k];
-    }
-    int incl = local;
+  for (int k = 0; k < 8; ++k) local += v[k];
+  int incl = local;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int n = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += n;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    if (t == 0) {
-      int acc = 0;
-      for (int w = 0; w < 16; ++w) {
-        int s = wsum[w];
-        wsum[w] = acc;
-        acc += s;
-      }
-      tile_total = acc;
-    }
-    __syncthreads();
-    int run = carry + wsum[wid] + incl - local;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (c0 + k < ncells) {
-        cs[c0 + k] = run;
-        count[c0 + k] = 0;
-      }
-      run += v[k];
-    }
-    carry += tile_total;
-    __syncthreads();
+  for (int off = 1; off < 64; off <<= 1) {
+    int n = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += n;
   }
-  if (t == 0) cs[ncells] = carry;
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int w = 0; w < 16; ++w) {
+      int s = wsum[w];
+      wsum[w] = acc;
+      acc += s;
+    }
+    tot = acc;
+  }
+  __syncthreads();
+  *total = tot;
+  return wsum[wid] + incl - local;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *tsum, int ncells) {
+  const int c0 = blockIdx.x * kScanTile + threadIdx.x * 8;
+  int v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = c0 + k < ncells ? count[c0 + k] : 0;
+  int total;
+  int run = block_scan8(v, &total);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (c0 + k < ncells) {
+      cs[c0 + k] = run;
+      count[c0 + k] = 0;
+    }
+    run += v[k];
+  }
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+// One workgroup: tile totals -> exclusive tile offsets (at most kScanTile
+// tiles), and the grand total into cs[ncells].
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(int *tsum, int ntiles, int *cs,
+                                                         int ncells) {
+  const int c0 = threadIdx.x * 8;
+  int v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = c0 + k < ntiles ? tsum[c0 + k] : 0;
+  int total;
+  int run = block_scan8(v, &total);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (c0 + k < ntiles) tsum[c0 + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) cs[ncells] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, int ncells) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells && c >= kScanTile) cs[c] += tsum[c / kScanTile];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const int *__restrict__ cell,
@@ -1310,27 +1337,35 @@ __global__ void k_coll_vv2(SimTable T, Bodies B, int ncoll) {
 // ---------------------------------------------------------------------------
 // kinetic energy: fixed partition + fixed reduction tree -> bitwise repeatable
 // ---------------------------------------------------------------------------
-// Block (b, s) sums particles b*kBlock + t + k*gridDim.x*kBlock of simulation s.
+// Partial sums of a simulation of n particles: np = min(kKeBlocks, n / kBlock
+// rounded up) blocks, block b summing particles b*kBlock + t + k*np*kBlock;
+// the partition depends on n only, so a simulation sums identically alone and
+// in a batch.
+__device__ __forceinline__ int n_parts(int n) { return min(kKeBlocks, (n + kBlock - 1) / kBlock); }
+
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_ke_partial(SimTable T, Particles<R> S, double *part) {
   const int sim = blockIdx.y;
   const int p0 = T.pbeg[sim], n = T.pbeg[sim + 1] - p0;
+  const int np = n_parts(n);
+  if ((int)blockIdx.x >= np) return;
   const bool d3 = T.P[sim].dim == 3;
   double acc[1] = {0.0};
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += np * kBlock) {
     const int i = p0 + k;
     double vx = S.v[0][i], vy = S.v[1][i], vz = d3 ? (double)S.v[2][i] : 0.0;
     acc[0] += 0.5 * (double)S.mass[i] * (vx * vx + vy * vy + vz * vz);
   }
   block_sum<1, kBlock>(acc);
-  if (threadIdx.x == 0) part[sim * gridDim.x + blockIdx.x] = acc[0];
+  if (threadIdx.x == 0) part[sim * kKeBlocks + blockIdx.x] = acc[0];
 }
 
-// Block s: the kinetic energy of simulation s from its n partial sums.
-__global__ __launch_bounds__(kBlock) void k_ke_final(const double *part, int n, double *out) {
+// Block s: the kinetic energy of simulation s from its partial sums.
+__global__ __launch_bounds__(kBlock) void k_ke_final(SimTable T, const double *part, double *out) {
   const int sim = blockIdx.x;
+  const int np = n_parts(T.pbeg[sim + 1] - T.pbeg[sim]);
   double acc[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += kBlock) acc[0] += part[sim * n + i];
+  for (int i = threadIdx.x; i < np; i += kBlock) acc[0] += part[sim * kKeBlocks + i];
   block_sum<1, kBlock>(acc);
   if (threadIdx.x == 0) out[sim] = acc[0];
 }
@@ -1359,8 +1394,10 @@ __global__ __launch_bounds__(kBlock) void k_polymer_partial(SimTable T, Particle
                                                             double *part) {
   const int sim = blockIdx.y;
   const int p0 = T.pbeg[sim], n = T.pbeg[sim + 1] - p0;
+  const int np = n_parts(n);
+  if ((int)blockIdx.x >= np) return;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += np * kBlock) {
     const int i = p0 + k;
     if (S.type[i] != 0) continue;
     acc[0] += (double)S.c[0][i] - (double)S.c[2][i];
@@ -1369,7 +1406,7 @@ __global__ __launch_bounds__(kBlock) void k_polymer_partial(SimTable T, Particle
   }
   block_sum<3, kBlock>(acc);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 3; ++k) part[3 * (sim * gridDim.x + blockIdx.x) + k] = acc[k];
+    for (int k = 0; k < 3; ++k) part[3 * (sim * kKeBlocks + blockIdx.x) + k] = acc[k];
 }
 
 // Block s: finishes the polymer sums of simulation s and its colloid chain
@@ -1377,14 +1414,14 @@ __global__ __launch_bounds__(kBlock) void k_polymer_partial(SimTable T, Particle
 // closer than the magnetic cut-off.  out[6 s + k]: [0] sum(cxx-cyy) [1] sum(cxy)
 // [2] fluid count [3] sum sin(2 phi) [4] sum cos(2 phi) [5] pair count.
 __global__ __launch_bounds__(kBlock) void k_observables_final(SimTable T, Bodies Ball,
-                                                              const double *part, int nparts,
-                                                              double *out) {
+                                                              const double *part, double *out) {
   const int sim = blockIdx.x;
   const sph_params_t &P = T.P[sim];
   const Bodies B = bodies_of(Ball, T, sim);
+  const int nparts = n_parts(T.pbeg[sim + 1] - T.pbeg[sim]);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nparts; b += kBlock)
-    for (int k = 0; k < 3; ++k) acc[k] += part[3 * (sim * nparts + b) + k];
+    for (int k = 0; k < 3; ++k) acc[k] += part[3 * (sim * kKeBlocks + b) + k];
   const int nc = P.N_colloids;
   const long long npairs = (long long)nc * nc;
   for (long long q = threadIdx.x; q < npairs; q += kBlock) {
@@ -1481,6 +1518,9 @@ class DeviceImpl final : public Device {
     }
     const int S = nsim();
     nc_ = P_.dim == 3 ? 6 : 3;
+    max_parts_ = 1;
+    for (const SimInfo &si : sims_)
+      max_parts_ = std::max(max_parts_, std::min(kKeBlocks, (si.n + kBlock - 1) / kBlock));
     if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (int b = 0; b < 2; ++b) {
       HIPCHK(x_[b].alloc(N_)); HIPCHK(y_[b].alloc(N_)); HIPCHK(z_[b].alloc(N_));
@@ -1494,7 +1534,12 @@ class DeviceImpl final : public Device {
     for (int a = 0; a < 9; ++a) HIPCHK(gv_[a].alloc(N_));
     HIPCHK(cell_.alloc(N_)); HIPCHK(slot_.alloc(N_)); HIPCHK(rank_.alloc(N_));
     HIPCHK(stage_src_.alloc(N_)); HIPCHK(stage_id_.alloc(N_)); HIPCHK(sorted_cell_.alloc(N_));
+    if (ncells_ > kScanTile * kScanTile) {
+      err = "too many cells in the batch (at most 8192 * 8192)";
+      return SPH_ERR_ARG;
+    }
     HIPCHK(count_.alloc(ncells_)); HIPCHK(cs_.alloc(ncells_ + 1));
+    HIPCHK(tsum_.alloc((ncells_ + kScanTile - 1) / kScanTile));
     HIPCHK(flag_.alloc(S));
     // Neighbour-list capacity: 1.5x the mean number of particles inside the
     // cut-off sphere (disc in 2D) at the initial number density, plus 64, for
@@ -1690,7 +1735,10 @@ class DeviceImpl final : public Device {
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
     k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
                                           flag_.p);
-    k_scan_cells<<<1, 1024, 0, stream_>>>(count_.p, cs_.p, ncells_);
+    const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
+    k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_);
+    k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_);
+    if (ntiles > 1) k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_);
     k_scatter<<<nb, kBlock, 0, stream_>>>(cell_.p, slot_.p, cs_.p, S.id, stage_src_.p,
                                           stage_id_.p, N_);
     k_rank_gather<R><<<nb, kBlock, 0, stream_>>>(S, T, cell_.p, cs_.p, stage_src_.p, stage_id_.p,
@@ -1900,10 +1948,9 @@ class DeviceImpl final : public Device {
 
   int observable_sums(double *out, std::string &err) override {
     const int S = nsim();
-    k_polymer_partial<R><<<dim3(kKeBlocks, S), kBlock, 0, stream_>>>(table(), parts(cur_),
-                                                                     obs_part_.p);
-    k_observables_final<<<S, kBlock, 0, stream_>>>(table(), bodies(), obs_part_.p, kKeBlocks,
-                                                   obs_out_.p);
+    k_polymer_partial<R><<<dim3(max_parts_, S), kBlock, 0, stream_>>>(table(), parts(cur_),
+                                                                       obs_part_.p);
+    k_observables_final<<<S, kBlock, 0, stream_>>>(table(), bodies(), obs_part_.p, obs_out_.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, obs_out_.p, 6 * S * sizeof(double), hipMemcpyDeviceToHost,
                           stream_));
@@ -1913,9 +1960,9 @@ class DeviceImpl final : public Device {
 
   int kinetic_energy(double *ke, std::string &err) override {
     const int S = nsim();
-    k_ke_partial<R><<<dim3(kKeBlocks, S), kBlock, 0, stream_>>>(table(), parts(cur_),
-                                                                ke_part_.p);
-    k_ke_final<<<S, kBlock, 0, stream_>>>(ke_part_.p, kKeBlocks, ke_out_.p);
+    k_ke_partial<R><<<dim3(max_parts_, S), kBlock, 0, stream_>>>(table(), parts(cur_),
+                                                                  ke_part_.p);
+    k_ke_final<<<S, kBlock, 0, stream_>>>(table(), ke_part_.p, ke_out_.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ke, ke_out_.p, S * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
@@ -2254,7 +2301,7 @@ class DeviceImpl final : public Device {
   int N_ = 0, ncells_ = 0, ncoll_ = 0, nclist_ = 0, nwlist_ = 0;
   bool any_walls_ = false;
   int nc_ = 3, cur_ = 0;
-  int block_ = kBlock, xcd_ = 1, nblk_ = 0;
+  int block_ = kBlock, xcd_ = 1, nblk_ = 0, max_parts_ = 1;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
   hipStream_t stream_ = nullptr;
   hipGraphExec_t exec_[2] = {nullptr, nullptr};
@@ -2267,7 +2314,7 @@ class DeviceImpl final : public Device {
   DevBuf<R> dens_, press_, dinv_, f_[3], dcdt_[6], gv_[9];
   DevBuf<PairRec<R>> rec_;
   bool rec_valid_ = false;
-  DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, flag_;
+  DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, tsum_, flag_;
   DevBuf<int4> nl_idx_;
   DevBuf<int> nl_count_, nl_queue_, nl_qlen_;
   int kmax_ = 0;
