@@ -71,6 +71,13 @@ struct Particles {  // per-particle state that moves with the reorder
   R *c[6];  // xx xy yy xz yz zz
   R *mass;
   int *type, *id;
+  // (x, y, z, 0) rounded to float, in the current sorted order: the FP32 form
+  // tests candidates against the cut-off with one 16-byte load each.  Kept
+  // current wherever positions change (reorder, VV1, colloid placement, uploads).
+  float4 *p4;
+  // (x, y, z, 0) in double, same order: one 32-byte access per gathered
+  // neighbour instead of three scattered 8-byte ones.
+  double4 *pd;
 };
 
 template <class R>
@@ -121,6 +128,32 @@ __device__ __forceinline__ int sim_of(const SimTable &T, int i) {
   }
   return lo;
 }
+
+// Verlet lists: every particle within rcut + skin at the last rebuild, in the
+// order of a 5 x 5 (x 5) cell scan.  Until any particle of a simulation has
+// moved more than skin / 2 from its position at the rebuild (ref), the list
+// still holds every pair closer than rcut, so the per-step passes test only the
+// list instead of 27 cells.  rebuild[s] asks for a rebuild of simulation s in
+// the current step; the reorder, list build and strip scans of the other
+// simulations are skipped (their sorted order stays put).
+struct VerletCtl {
+  int *rebuild;        // [nsim] 1: rebuild this step (set by the displacement checks)
+  int *overflow;       // [nsim] a list overflowed at the last build
+  float4 *ref;         // [N] positions at the last rebuild
+  float half_skin_frac;  // (skin / 2) / rcut, slightly reduced for rounding
+  int on;              // displacement checks active
+};
+
+struct VList {
+  int4 *idx;   // (cap / 4) * N groups, column-major like NList
+  int *count;  // N: entries found at the build (may exceed cap)
+  // Per step, the density pass marks which entries are inside rcut: bit e % 32
+  // of word e / 32 (column-major, words * N), so the pair pass visits exactly
+  // the neighbours without a second index list.
+  unsigned *mask;
+  int cap;
+  float skin_frac;  // skin / rcut
+};
 
 // The bodies of simulation s: colloid arrays shifted to its first colloid, so
 // a particle of type 3 + k addresses colloid k of its own simulation.
@@ -262,6 +295,53 @@ __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int 
   }
 }
 
+// The 5 x 5 (x 5) cell neighbourhood of the Verlet build: calls fn(k0, k1, sx,
+// sy, sz) for each contiguous sorted range of rows y-2..y+2 (z-2..z+2) and
+// cells x-2..x+2, split at the periodic seam as for_each_strip does.  Needs at
+// least 5 cells along every periodic axis (checked on the host), so no cell is
+// visited twice.
+template <int DIM, bool WALL, class Fn>
+__device__ __forceinline__ void for_each_strip5(const sph_params_t &P, const int *__restrict__ cs,
+                                                int cx, int cy, int cz, Fn &&fn) {
+  const int Nx = P.Ncells[0], Ny = P.Ncells[1], Nz = P.Ncells[2];
+  if (cx < 0 || cx > Nx || cy < -1 || cy > Ny || (DIM == 3 && (cz < 0 || cz > Nz))) return;
+  const double Lx = P.L[0];
+  int a0 = cx - 2, a1 = cx + 3, b0 = 0, b1 = 0;
+  double sa = 0.0, sb = 0.0;
+  if (a0 < 0) {          // [Nx + a0, Nx) shifted +L, then [0, a1)
+    b1 = a1;
+    a1 = Nx;
+    a0 = Nx + a0;
+    sa = Lx;
+  } else if (a1 > Nx) {  // [a0, Nx), then [0, a1 - Nx) shifted -L
+    b1 = a1 - Nx;
+    a1 = Nx;
+    sb = -Lx;
+  }
+  constexpr int kRows = DIM == 3 ? 25 : 5;
+#pragma unroll 1
+  for (int s = 0; s < 2 * kRows; ++s) {
+    const int part = s & 1;
+    if (part && b1 == 0) continue;
+    const int row = s >> 1;
+    int y = cy + row % 5 - 2;
+    int z = DIM == 3 ? cz + row / 5 - 2 : 0;
+    double sy = 0.0, sz = 0.0;
+    if (DIM == 3) {
+      if (z < 0) { z += Nz; sz = P.L[2]; }
+      if (z >= Nz) { z -= Nz; sz = -P.L[2]; }
+    }
+    if (!WALL) {
+      if (y < 0) { y += Ny; sy = P.L[1]; }
+      if (y >= Ny) { y -= Ny; sy = -P.L[1]; }
+    } else if (y < 0 || y >= Ny) {
+      continue;
+    }
+    const int base = (z * Ny + y) * Nx;
+    fn(cs[base + (part ? b0 : a0)], cs[base + (part ? b1 : a1)], part ? sb : sa, sy, sz);
+  }
+}
+
 // r_ij of a candidate inside a strip segment: (x_i - x_j) + s in double, rounded to R.
 template <class R, int DIM>
 __device__ __forceinline__ void seg_sep(double xi, double yi, double zi, double xj, double yj,
@@ -270,6 +350,66 @@ __device__ __forceinline__ void seg_sep(double xi, double yi, double zi, double 
   r[1] = (R)((yi - yj) + sy);
   r[2] = DIM == 3 ? (R)((zi - zj) + sz) : R(0);
 }
+
+// Cut-off test of a candidate in a strip segment.  FP64 form: (x_i - x_j) + s in
+// double, as the reference.  FP32 form: the same in float on the float copies
+// of the positions (rounding them moves r by ~1e-7 of the box, so only pairs
+// within that distance of the cut-off - where W and gradW vanish to fifth and
+// fourth order - can fall on the other side).  Both forms give a pure function
+// of the positions, identical in every pass that re-scans the strips.
+template <class R, int DIM>
+struct Cand;
+template <int DIM>
+struct Cand<double, DIM> {
+  double x, y, z;
+  __device__ __forceinline__ void load(const Particles<double> &S, int k) {
+    x = S.x[k];
+    y = S.y[k];
+    z = DIM == 3 ? S.z[k] : 0.0;
+  }
+};
+template <int DIM>
+struct Cand<float, DIM> {
+  float4 q;
+  __device__ __forceinline__ void load(const Particles<float> &S, int k) { q = S.p4[k]; }
+};
+template <class R, int DIM>
+struct CandSelf;
+template <int DIM>
+struct CandSelf<double, DIM> {
+  double x, y, z, sx, sy, sz;
+  __device__ __forceinline__ void init(const Particles<double> &S, int i) {
+    x = S.x[i];
+    y = S.y[i];
+    z = DIM == 3 ? S.z[i] : 0.0;
+  }
+  __device__ __forceinline__ void shift(double a, double b, double c) { sx = a; sy = b; sz = c; }
+  __device__ __forceinline__ double r2(const Cand<double, DIM> &c) const {
+    double r[3];
+    seg_sep<double, DIM>(x, y, z, c.x, c.y, c.z, sx, sy, sz, r);
+    return DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+  }
+};
+template <int DIM>
+struct CandSelf<float, DIM> {
+  float x, y, z, sx, sy, sz;
+  __device__ __forceinline__ void init(const Particles<float> &S, int i) {
+    const float4 q = S.p4[i];
+    x = q.x;
+    y = q.y;
+    z = q.z;
+  }
+  __device__ __forceinline__ void shift(double a, double b, double c) {
+    sx = (float)a;
+    sy = (float)b;
+    sz = (float)c;
+  }
+  __device__ __forceinline__ float r2(const Cand<float, DIM> &c) const {
+    const float rx = (x - c.q.x) + sx, ry = (y - c.q.y) + sy;
+    const float rz = DIM == 3 ? (z - c.q.z) + sz : 0.0f;
+    return DIM == 3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+  }
+};
 
 // r_ij = x_i - x_j with the minimum image (x, z, and y without walls), computed
 // in double and rounded to R.
@@ -543,13 +683,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, i
   if (c < ncells && c >= kScanTile) cs[c] += tsum[c / kScanTile];
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(const int *__restrict__ cell,
+// rebuild (when not null): only simulations with rebuild[s] set are reordered.
+__global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__restrict__ cell,
                                                     const int *__restrict__ slot,
                                                     const int *__restrict__ cs,
                                                     const int *__restrict__ id, int *stage_src,
-                                                    int *stage_id, int N) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= N) return;
+                                                    int *stage_id, const int *rebuild) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const int k = T.blk_first[b] + threadIdx.x;
+  if (k >= T.pbeg[sim + 1] || (rebuild && !rebuild[sim])) return;
   int c = cell[k];
   if (c < 0) return;
   int p = cs[c] + slot[k];
@@ -559,13 +701,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int *__restrict__ cell
 
 // Final position = cell start + number of particles of the same cell with a
 // smaller ID, i.e. the stable (cell, ID) order.  Gathers the moving state.
+// A simulation that is not rebuilt keeps its order: its state is copied
+// position for position into the other buffer set (the step always flips).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
-    Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
+    SimTable T, Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
     const int *__restrict__ cs, const int *__restrict__ stage_src,
-    const int *__restrict__ stage_id, int *rank, int *sorted_cell, int ncells, int nc, int N) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N || p >= cs[ncells]) return;
+    const int *__restrict__ stage_id, int *rank, int *sorted_cell, int nc,
+    const int *rebuild, int *newpos) {
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const int p = T.blk_first[b] + threadIdx.x;
+  if (p >= T.pbeg[sim + 1]) return;
+  if (rebuild && !rebuild[sim]) {
+    dst.x[p] = src.x[p];
+    dst.y[p] = src.y[p];
+    dst.z[p] = src.z[p];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dst.v[a][p] = src.v[a][p];
+    for (int a = 0; a < nc; ++a) dst.c[a][p] = src.c[a][p];
+    dst.mass[p] = src.mass[p];
+    dst.type[p] = src.type[p];
+    dst.id[p] = src.id[p];
+    return;
+  }
   const int k = stage_src[p];
   const int c = cell[k];
   const int s0 = cs[c], s1 = cs[c + 1];
@@ -573,9 +731,12 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   int r = 0;
   for (int q = s0; q < s1; ++q) r += stage_id[q] < myid;
   const int d = s0 + r;
-  dst.x[d] = src.x[k];
-  dst.y[d] = src.y[k];
-  dst.z[d] = src.z[k];
+  const double xk = src.x[k], yk = src.y[k], zk = src.z[k];
+  dst.x[d] = xk;
+  dst.y[d] = yk;
+  dst.z[d] = zk;
+  dst.p4[d] = make_float4((float)xk, (float)yk, (float)zk, 0.0f);
+  dst.pd[d] = make_double4(xk, yk, zk, 0.0);
 #pragma unroll
   for (int a = 0; a < 3; ++a) dst.v[a][d] = src.v[a][k];
   for (int a = 0; a < nc; ++a) dst.c[a][d] = src.c[a][k];
@@ -584,15 +745,24 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   dst.id[d] = myid;
   rank[myid] = d;
   sorted_cell[d] = c;
+  if (newpos) newpos[k] = d;
+}
+
+// tmp[newpos[k]] = a[k]: a per-particle field carried into the new order.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_permute(const T *__restrict__ a, T *__restrict__ tmp,
+                                                    const int *__restrict__ newpos, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) tmp[newpos[k]] = a[k];
 }
 
 // ---------------------------------------------------------------------------
 // pair pass 1: density (+ pressure) and the neighbour list
 // ---------------------------------------------------------------------------
-// Scans the candidate strips once.  Every candidate inside the cut-off adds
-// W(r) to the density (self included, src/kernel_densities.cu:95-99) and, when
-// it is not the particle itself, is appended to the particle's neighbour list
-// in scan order.  The list is column-major - entry k of particle i sits at
+// Scans the candidate strips once, appending every other particle inside the
+// cut-off to the particle's neighbour list in scan order; then sums W(r) over
+// the particle itself and its list (src/kernel_densities.cu:16-193 sums the
+// same set in cell order).  The list is column-major - entry k of particle i sits at
 // nlist[k * N + i] - so a wavefront's 64 lanes write and read 256 contiguous
 // bytes per entry.  `count` holds the true number of neighbours even past kmax;
 // the pair pass re-scans the strips for the entries that did not fit.
@@ -673,7 +843,7 @@ __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Part
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, Derived<R> D,
                                                     PairRec<R> *rec, const int *__restrict__ csg,
-                                                    NList L, int write_dens) {
+                                                    NList L, int write_dens, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int *__restrict__ cs = csg + T.cell_off[sim];
@@ -690,35 +860,192 @@ __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, 
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
   const R rcutsq = (R)P.rcutsq, rinv = (R)P.rcut_inv, cw = (R)P.cw;
   const int kmax = L.kmax;
-  R d = R(0);
   int cnt = 0;
-  int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
-  for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
-                                                    double sz) {
-    for (int k = k0; k < k1; ++k) {
-      R r[3];
-      seg_sep<R, DIM>(xi, yi, zi, X[k], Y[k], DIM == 3 ? Z[k] : 0.0, sx, sy, sz, r);
-      R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-      if (r2 < rcutsq) {
-        d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
-        if (k != i) {
-          buf = make_int4(buf.y, buf.z, buf.w, k);
+  R d = R(0);
+  if (VL.idx && VL.count[i] <= VL.cap) {
+    // Verlet list: the hits in list order, W summed as they come (most list
+    // entries are hits, so lanes stay converged); the hits are marked in the
+    // mask words, stored as each fills (all lanes store word w together).
+    d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+    unsigned mw = 0u;
+    const int vc = VL.count[i];
+    const int4 *__restrict__ vl = VL.idx + i;
+#pragma unroll 1
+    for (int g = 0; g < vc; g += 4) {
+      const int4 q = vl[(size_t)(g >> 2) * N];
+      const int m = min(4, vc - g);
+      const int jj[4] = {q.x, q.y, q.z, q.w};
+      double4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pj[u] = S.pd[u < m ? jj[u] : i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (u >= m) break;
+        R r[3];
+        pair_sep<R, DIM, WALL>(P, xi, yi, zi, pj[u].x, pj[u].y, DIM == 3 ? pj[u].z : 0.0, r);
+        const R r2 =
+            DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+        if (r2 < rcutsq) {
+          d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+          mw |= 1u << ((g + u) & 31);
           cnt++;
-          if ((cnt & 3) == 0 && cnt <= kmax) out[(size_t)((cnt >> 2) - 1) * N] = buf;
         }
       }
+      if (((g + 4) & 31) == 0 || g + 4 >= vc) {
+        VL.mask[(size_t)(g >> 5) * N + i] = mw;
+        mw = 0u;
+      }
     }
-  });
-  const int tail = cnt & 3;
-  if (tail && cnt < kmax) {
-    for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
-    out[(size_t)(cnt >> 2) * N] = buf;
+  } else {
+    R d_over = R(0);  // W of the hits past the list capacity, in scan order
+    int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
+    // Phase 1: the candidate test only.  Four candidates' positions are loaded
+    // before any is tested, so their latencies overlap.
+    CandSelf<R, DIM> me;
+    me.init(S, i);
+    auto test = [&](int k, const Cand<R, DIM> &c) {
+      const R r2 = (R)me.r2(c);
+      if (r2 < rcutsq && k != i) {
+        buf = make_int4(buf.y, buf.z, buf.w, k);
+        cnt++;
+        if (cnt > kmax) d_over = d_over + sph_W<R>(pair_sqrt(r2), rinv, cw);
+        else if ((cnt & 3) == 0) out[(size_t)((cnt >> 2) - 1) * N] = buf;
+      }
+    };
+    for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
+                                                      double sz) {
+      me.shift(sx, sy, sz);
+      int k = k0;
+#pragma unroll 1
+      for (; k + 4 <= k1; k += 4) {
+        Cand<R, DIM> c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u].load(S, k + u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) test(k + u, c[u]);
+      }
+#pragma unroll 1
+      for (; k < k1; ++k) {
+        Cand<R, DIM> c;
+        c.load(S, k);
+        test(k, c);
+      }
+    });
+    const int tail = cnt & 3;
+    if (tail && cnt < kmax) {
+      for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
+      out[(size_t)(cnt >> 2) * N] = buf;
+    }
+    // Phase 2: W over the stored hits with converged lanes (in phase 1 a wave
+    // would evaluate W under a mask for every candidate any lane hits).  The
+    // minimum image reproduces the strip segment's shift bit for bit.
+    d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+    if (write_dens) {
+      const int stored = min(cnt, kmax);
+#pragma unroll 1
+      for (int g = 0; g < stored; g += 4) {
+        int4 q = out[(size_t)(g >> 2) * N];
+        const int m = min(4, stored - g);
+        double xj[4], yj[4], zj[4];
+        const int jj[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = u < m ? jj[u] : i;
+          xj[u] = X[j];
+          yj[u] = Y[j];
+          zj[u] = DIM == 3 ? Z[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (u >= m) break;
+          R r[3];
+          pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj[u], yj[u], zj[u], r);
+          const R r2 =
+              DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+          d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+        }
+      }
+      d = d + d_over;
+    }
   }
   L.count[i] = cnt;
   if (write_dens) {
     D.dens[i] = d;
     pressure_terms<R>(P, S, D, rec, i, d);
   }
+}
+
+// Verlet list build of the simulations being rebuilt: every other particle
+// within rcut + skin over the 5 x 5 (x 5) cell neighbourhood (cells are at
+// least rcut wide and skin <= rcut), in scan order; the positions at the build
+// become the reference of the displacement checks.  Lists that overflow their
+// capacity raise overflow[sim]: those particles use the strip scan, and the
+// simulation is rebuilt again next step.
+template <int DIM, bool WALL, class R>
+__global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R> S,
+                                                         const int *__restrict__ csg, VList VL,
+                                                         VerletCtl V) {
+  const int b = tile_of(T), sim = T.blk_sim[b];
+  if (!V.rebuild[sim]) return;
+  const int i = T.blk_first[b] + threadIdx.x;
+  if (i >= T.pbeg[sim + 1]) return;
+  const sph_params_t &P = T.P[sim];
+  const int *__restrict__ cs = csg + T.cell_off[sim];
+  const int N = T.ntotal;
+  const double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
+  int cx, cy, cz;
+  cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
+  const double rv = P.rcut * (1.0 + (double)VL.skin_frac), rv2 = rv * rv;
+  int4 *__restrict__ out = VL.idx + i;
+  const int cap = VL.cap;
+  int cnt = 0;
+  int4 buf = make_int4(0, 0, 0, 0);
+  for_each_strip5<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
+                                                     double sz) {
+    int k = k0;
+#pragma unroll 1
+    for (; k < k1; ++k) {
+      double r[3];
+      seg_sep<double, DIM>(xi, yi, zi, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, sx, sy, sz, r);
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (r2 < rv2 && k != i) {
+        buf = make_int4(buf.y, buf.z, buf.w, k);
+        cnt++;
+        if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
+      }
+    }
+  });
+  const int tail = cnt & 3;
+  if (tail && cnt < cap) {
+    for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
+    out[(size_t)(cnt >> 2) * N] = buf;
+  }
+  VL.count[i] = cnt;
+  V.ref[i] = S.p4[i];
+  if (cnt > cap) atomicOr(&V.overflow[sim], 1);
+}
+
+// After a build: simulations whose lists overflowed are rebuilt again next step.
+__global__ void k_verlet_done(VerletCtl V, int nsim) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsim) return;
+  V.rebuild[s] = V.overflow[s] ? 1 : 0;
+  V.overflow[s] = 0;
+}
+
+__global__ void k_fill(int *a, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Float copies of the positions (after an upload of x, y or z).
+__global__ __launch_bounds__(kBlock) void k_pack_pos(const double *x, const double *y,
+                                                     const double *z, float4 *p4, double4 *pd,
+                                                     int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p4[i] = make_float4((float)x[i], (float)y[i], (float)z[i], 0.0f);
+  pd[i] = make_double4(x[i], y[i], z[i], 0.0);
 }
 
 template <class R>
@@ -947,7 +1274,7 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
     SimTable T, Particles<R> S, Derived<R> D, Bodies Ball, const PairRec<R> *__restrict__ rec,
-    NList L, int store_gv, int write_dcdt, int with_ext) {
+    NList L, int store_gv, int write_dcdt, int with_ext, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const int i = T.blk_first[b] + threadIdx.x;
   if (i >= T.pbeg[sim + 1]) return;
@@ -962,6 +1289,26 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
     acc.f[a] = R(0);
 #pragma unroll
     for (int b = 0; b < 3; ++b) acc.g[a][b] = R(0);
+  }
+  if (VL.idx && VL.count[i] <= VL.cap) {
+    // the Verlet entries the density pass marked, in list order; all lanes walk
+    // the list groups together so every index load is coalesced
+    const int vc = VL.count[i];
+    unsigned m = 0u;
+#pragma unroll 1
+    for (int g = 0; g < vc; g += 4) {
+      if ((g & 31) == 0) m = VL.mask[(size_t)(g >> 5) * N + i];
+      int4 q = VL.idx[(size_t)(g >> 2) * N + i];
+      unsigned nib = (m >> (g & 31)) & 0xFu;
+#pragma unroll 1
+      for (int u = 0; u < 4; ++u) {
+        if (nib & 1u) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, q.x, acc);
+        nib >>= 1;
+        q = make_int4(q.y, q.z, q.w, 0);
+      }
+    }
+    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+    return;
   }
   const int total = L.count[i];
   const int cnt = min(total, L.kmax);
@@ -1015,17 +1362,20 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
 #pragma unroll
       for (int b = 0; b < 3; ++b) acc.g[a][b] = D.gv[a * 3 + b][i];
     }
-    int cx, cy, cz;
-    cell_coords<DIM, WALL>(P, me.x, me.y, me.z, cx, cy, cz);
     const R rcutsq = (R)P.rcutsq;
     const int skip = L.kmax;
     int hit = 0;
+    int cx, cy, cz;
+    cell_coords<DIM, WALL>(P, me.x, me.y, me.z, cx, cy, cz);
+    CandSelf<R, DIM> cme;
+    cme.init(S, i);
     for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
                                                       double sz) {
+      cme.shift(sx, sy, sz);
       for (int k = k0; k < k1; ++k) {
-        R r[3];
-        seg_sep<R, DIM>(me.x, me.y, me.z, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, sx, sy, sz, r);
-        R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+        Cand<R, DIM> c;
+        c.load(S, k);
+        const R r2 = (R)cme.r2(c);
         if (r2 < rcutsq && k != i) {
           if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, k, acc);
           hit++;
@@ -1151,8 +1501,24 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
 // ---------------------------------------------------------------------------
 // integrators
 // ---------------------------------------------------------------------------
+// Raises rebuild[sim] when the particle at sorted position i has moved more than
+// skin / 2 from its position at the last rebuild (minimum image; float).
+__device__ __forceinline__ void displacement_check(const VerletCtl &V, const sph_params_t &P,
+                                                   int sim, int i, double x, double y, double z) {
+  if (!V.on) return;
+  const float4 r = V.ref[i];
+  float dx = (float)x - r.x, dy = (float)y - r.y, dz = P.dim == 3 ? (float)z - r.z : 0.0f;
+  const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
+  dx -= Lx * rintf(dx / Lx);
+  if (!P.wall) dy -= Ly * rintf(dy / Ly);
+  if (P.dim == 3) dz -= Lz * rintf(dz / Lz);
+  const float lim = V.half_skin_frac * (float)P.rcut;
+  if (dx * dx + dy * dy + dz * dz > lim * lim) atomicOr(&V.rebuild[sim], 1);
+}
+
 template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Derived<R> D) {
+__global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Derived<R> D,
+                                                VerletCtl V) {
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
@@ -1189,6 +1555,9 @@ __global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Deri
   S.x[i] = xi;
   S.y[i] = yi;
   if (DIM == 3) S.z[i] = zi;
+  S.p4[i] = make_float4((float)xi, (float)yi, (float)zi, 0.0f);
+  S.pd[i] = make_double4(xi, yi, zi, 0.0);
+  displacement_check(V, P, sim, i, xi, yi, zi);
 }
 
 template <class R, int DIM>
@@ -1277,11 +1646,13 @@ __global__ void k_coll_vv1(SimTable T, Bodies B, int ncoll) {
 // Rigidly re-places every colloid boundary particle (one thread per list entry).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_coll_place(SimTable T, Particles<R> S, Bodies B,
-                                                       const int *__restrict__ rank, int nlist) {
+                                                       const int *__restrict__ rank, int nlist,
+                                                       VerletCtl V) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nlist) return;
   const int ci = B.cowner[j];  // colloid of the batch
-  const sph_params_t &P = T.P[T.coll_sim[ci]];
+  const int sim = T.coll_sim[ci];
+  const sph_params_t &P = T.P[sim];
   if (P.coll_move != 0) return;
   double *const *C = B.s;
   double px, py, pz = 0.0;
@@ -1315,6 +1686,9 @@ __global__ __launch_bounds__(kBlock) void k_coll_place(SimTable T, Particles<R> 
   S.x[p] = X;
   S.y[p] = Y;
   if (P.dim == 3) S.z[p] = Z;
+  S.p4[p] = make_float4((float)X, (float)Y, (float)Z, 0.0f);
+  S.pd[p] = make_double4(X, Y, Z, 0.0);
+  displacement_check(V, P, sim, p, X, Y, Z);
 }
 
 __global__ void k_coll_vv2(SimTable T, Bodies B, int ncoll) {
@@ -1528,6 +1902,8 @@ class DeviceImpl final : public Device {
       for (int a = 0; a < 6; ++a) HIPCHK(c_[b][a].alloc(N_));
       HIPCHK(mass_[b].alloc(N_)); HIPCHK(type_[b].alloc(N_)); HIPCHK(id_[b].alloc(N_));
     }
+    HIPCHK(p4_.alloc(N_)); HIPCHK(pd_.alloc(N_));
+    HIPCHK(newpos_.alloc(N_)); HIPCHK(ptmp_.alloc(N_));
     HIPCHK(dens_.alloc(N_)); HIPCHK(press_.alloc(N_));
     for (int a = 0; a < 3; ++a) HIPCHK(f_[a].alloc(N_));
     for (int a = 0; a < 6; ++a) HIPCHK(dcdt_[a].alloc(N_));
@@ -1559,6 +1935,38 @@ class DeviceImpl final : public Device {
     if (const char *e = std::getenv("SPH_NEIGHBOUR_CAPACITY")) kmax_ = std::max(1, std::atoi(e));
     kmax_ = (kmax_ + 3) / 4 * 4;
     HIPCHK(nl_idx_.alloc((size_t)(kmax_ / 4) * N_));
+    // Verlet lists: skin as a fraction of rcut (SPH_VERLET_SKIN, 0 disables),
+    // capacity 1.5x the mean count inside rcut + skin plus 64.
+    // 0.05 rcut: measured best on the 1M-particle walls + colloids flow (lists
+    // 8% longer than the neighbour set; a rebuild every ~100 steps there).
+    skin_frac_ = 0.05f;
+    if (const char *e = std::getenv("SPH_VERLET_SKIN")) skin_frac_ = (float)std::atof(e);
+    verlet_ok_ = skin_frac_ > 0.0f && skin_frac_ <= 1.0f;
+    double vmean = 0.0;
+    for (const SimInfo &si : sims_) {
+      const sph_params_t &p = si.p;
+      if (p.Ncells[0] < 5 || (!p.wall && p.Ncells[1] < 5) || (p.dim == 3 && p.Ncells[2] < 5))
+        verlet_ok_ = false;
+      double vol = p.L[0] * (p.L[1] + (p.wall ? 2.0 * p.wall_width : 0.0)) *
+                   (p.dim == 3 ? p.L[2] : 1.0);
+      const double rv = p.rcut * (1.0 + skin_frac_);
+      vmean = std::max(vmean, p.N / vol * (p.dim == 3 ? 4.0 / 3.0 * M_PI * rv * rv * rv
+                                                       : M_PI * rv * rv));
+    }
+    vcap_ = ((int)(1.5 * vmean) + 64 + 3) / 4 * 4;
+    if (const char *e = std::getenv("SPH_VERLET_CAPACITY"))
+      vcap_ = std::max(4, (std::atoi(e) + 3) / 4 * 4);
+    if (verlet_ok_) {
+      HIPCHK(vl_idx_.alloc((size_t)(vcap_ / 4) * N_));
+      HIPCHK(vl_mask_.alloc((size_t)((vcap_ + 31) / 32) * N_));
+      HIPCHK(vl_count_.alloc(N_));
+      HIPCHK(ref4_.alloc(N_));
+      HIPCHK(vrebuild_.alloc(S));
+      HIPCHK(voverflow_.alloc(S));
+      HIPCHK(hipMemset(vrebuild_.p, 0, S * sizeof(int)));
+      HIPCHK(hipMemset(voverflow_.p, 0, S * sizeof(int)));
+    }
+    verlet_valid_ = verlet_dirty_ = in_verlet_ = false;
     HIPCHK(nl_count_.alloc(N_));
     HIPCHK(nl_queue_.alloc(N_));
     HIPCHK(nl_qlen_.alloc(1));
@@ -1594,6 +2002,15 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemcpy(x_[0].p, xyz[0].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(y_[0].p, xyz[1].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(z_[0].p, xyz[2].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
+    {
+      std::vector<float4> q(N_);
+      for (int i = 0; i < N_; ++i)
+        q[i] = make_float4((float)xyz[0][i], (float)xyz[1][i], (float)xyz[2][i], 0.0f);
+      HIPCHK(hipMemcpy(p4_.p, q.data(), N_ * sizeof(float4), hipMemcpyHostToDevice));
+      std::vector<double4> qd(N_);
+      for (int i = 0; i < N_; ++i) qd[i] = make_double4(xyz[0][i], xyz[1][i], xyz[2][i], 0.0);
+      HIPCHK(hipMemcpy(pd_.p, qd.data(), N_ * sizeof(double4), hipMemcpyHostToDevice));
+    }
     auto put_all = [&](R *dst, int field) -> int {
       for (int s = 0; s < S; ++s) {
         const HostState &h = *hs[s];
@@ -1730,8 +2147,15 @@ class DeviceImpl final : public Device {
 
   // ---- per-step phases ----
   int neighbours_search(std::string &err) override {
+    verlet_valid_ = false;
+    return reorder(nullptr, true, err);
+  }
+
+  // Hash, scan, reorder.  With `gate`, simulations whose gate[s] is 0 keep
+  // their order (copied into the other buffer set).  `carry` moves the derived
+  // fields along (not needed inside a step, which recomputes them).
+  int reorder(const int *gate, bool carry, std::string &err) {
     Mark m(this, PH_NS);
-    const int nb = blocks(N_);
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
     k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
                                           flag_.p);
@@ -1739,11 +2163,27 @@ class DeviceImpl final : public Device {
     k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_);
     k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_);
     if (ntiles > 1) k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_);
-    k_scatter<<<nb, kBlock, 0, stream_>>>(cell_.p, slot_.p, cs_.p, S.id, stage_src_.p,
-                                          stage_id_.p, N_);
-    k_rank_gather<R><<<nb, kBlock, 0, stream_>>>(S, T, cell_.p, cs_.p, stage_src_.p, stage_id_.p,
-                                                 rank_.p, sorted_cell_.p, ncells_, nc_, N_);
+    k_scatter<<<nblk_, block_, 0, stream_>>>(table(), cell_.p, slot_.p, cs_.p, S.id,
+                                             stage_src_.p, stage_id_.p, gate);
+    k_rank_gather<R><<<nblk_, block_, 0, stream_>>>(table(), S, T, cell_.p, cs_.p, stage_src_.p,
+                                                    stage_id_.p, rank_.p, sorted_cell_.p, nc_,
+                                                    gate, carry ? newpos_.p : nullptr);
     HIPCHK(hipGetLastError());
+    if (carry) {
+      // A stand-alone neighbour search: the fields computed before it (density,
+      // pressure, forces, dc/dt, gradient) follow their particles, so any
+      // method sequence reads them in the new order.  (Inside a step they are
+      // all recomputed after the reorder.)
+      std::vector<R *> fields = {dens_.p, press_.p, dinv_.p};
+      for (int a = 0; a < 3; ++a) fields.push_back(f_[a].p);
+      for (int a = 0; a < 6; ++a) fields.push_back(dcdt_[a].p);
+      for (int a = 0; a < 9; ++a) fields.push_back(gv_[a].p);
+      for (R *f : fields) {
+        k_permute<R><<<blocks(N_), kBlock, 0, stream_>>>(f, ptmp_.p, newpos_.p, N_);
+        HIPCHK(hipMemcpyAsync(f, ptmp_.p, N_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+      }
+      HIPCHK(hipGetLastError());
+    }
     cur_ ^= 1;
     list_valid_ = false;
     rec_valid_ = false;
@@ -1755,13 +2195,25 @@ class DeviceImpl final : public Device {
   // calculate_density: density, pressure and the neighbour list.
   int density(std::string &err) override { return build_list(1, err); }
 
+  // After steps with Verlet lists the sorted order of a simulation is that of
+  // its last rebuild, not of the current cells; a method that scans cells
+  // first re-sorts (a full neighbour search: same results, current order).
+  int settle(std::string &err) {
+    if (in_verlet_ || !verlet_dirty_) return SPH_OK;
+    verlet_dirty_ = false;
+    return neighbours_search(err);
+  }
+
   int build_list(int write_dens, std::string &err) {
+    int rc = settle(err);
+    if (rc) return rc;
     Mark m(this, PH_DENS);
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_density<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(),
-                                                              rec_.p, cs_.p, nlist(), write_dens);
+                                                              rec_.p, cs_.p, nlist(), write_dens,
+                                                              vlist());
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
@@ -1783,7 +2235,9 @@ class DeviceImpl final : public Device {
 
   template <bool GV, bool F>
   int pairs(int store_gv, int write_dcdt, int with_ext, int phase, std::string &err) {
-    int rc = ensure_list(err);
+    int rc = settle(err);
+    if (rc) return rc;
+    rc = ensure_list(err);
     if (rc) return rc;
     // Records hold positions, velocities and stresses: refresh them if any of
     // those changed since the last density/pressure pass.
@@ -1794,7 +2248,7 @@ class DeviceImpl final : public Device {
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<nblk_, block_, 0, stream_>>>(
           table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), store_gv, write_dcdt,
-          with_ext);
+          with_ext, vlist());
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
           table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), cs_.p, store_gv,
           write_dcdt, with_ext);
@@ -1853,19 +2307,21 @@ class DeviceImpl final : public Device {
   }
 
   int vv1(std::string &err) override {
+    if (!in_verlet_) verlet_valid_ = false;
     Mark m(this, PH_VV1);
     list_valid_ = false;
     rec_valid_ = false;
+    const VerletCtl V = vctl();
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_vv1<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived());
+      k_vv1<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(), V);
     });
     if (ncoll_ > 0) {
       k_coll_vv1<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
       if (nclist_ > 0)
         k_coll_place<R><<<blocks(nclist_), kBlock, 0, stream_>>>(table(), parts(cur_), bodies(),
-                                                                 rank_.p, nclist_);
+                                                                 rank_.p, nclist_, V);
     }
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -1894,8 +2350,36 @@ class DeviceImpl final : public Device {
   // counter, -1 advances the device counter (captured form).
   int one_step(int step, std::string &err) {
     int rc;
+    if (verlet_ok_) {
+      in_verlet_ = true;
+      rc = one_step_verlet(step, err);
+      in_verlet_ = false;
+      return rc;
+    }
     if ((rc = vv1(err))) return rc;
-    if ((rc = neighbours_search(err))) return rc;
+    if ((rc = reorder(nullptr, false, err))) return rc;
+    if ((rc = set_step(step, err))) return rc;
+    if ((rc = forces_all(false, err))) return rc;
+    return vv2(err);
+  }
+
+  // The same step with Verlet lists: the displacement checks of VV1 decide, per
+  // simulation, whether this step reorders and rebuilds its lists.
+  int one_step_verlet(int step, std::string &err) {
+    int rc;
+    if ((rc = vv1(err))) return rc;
+    if ((rc = reorder(vrebuild_.p, false, err))) return rc;
+    {
+      Mark m(this, PH_NS);
+      dispatch3([&](auto dimc, auto wallc) {
+        constexpr int DIM = decltype(dimc)::value;
+        constexpr bool WALL = decltype(wallc)::value;
+        k_verlet_build<DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), cs_.p,
+                                                                 vlist(), vctl());
+      });
+      k_verlet_done<<<blocks(nsim()), kBlock, 0, stream_>>>(vctl(), nsim());
+      HIPCHK(hipGetLastError());
+    }
     if ((rc = set_step(step, err))) return rc;
     if ((rc = forces_all(false, err))) return rc;
     return vv2(err);
@@ -1903,6 +2387,14 @@ class DeviceImpl final : public Device {
 
   int run_steps(int nsteps, int first_step, bool graph, std::string &err) override {
     int rc;
+    if (verlet_ok_ && nsteps > 0) {
+      if (!verlet_valid_) {  // lists of a different order or positions: rebuild all
+        k_fill<<<blocks(nsim()), kBlock, 0, stream_>>>(vrebuild_.p, nsim(), 1);
+        HIPCHK(hipGetLastError());
+      }
+      verlet_valid_ = true;
+      verlet_dirty_ = true;
+    }
     if (!graph || timing_) {
       for (int n = 0; n < nsteps; ++n)
         if ((rc = one_step(first_step + n, err))) return rc;
@@ -2067,6 +2559,7 @@ class DeviceImpl final : public Device {
     for (int k = 0; k < F_NUM; ++k) {
       if (name != kPFieldNames[k]) continue;
       if (k == F_X || k == F_Y || k == F_Z) {
+        verlet_valid_ = false;
         std::vector<double> tmp(n);
         for (int p = 0; p < n; ++p) {
           const int l = local_id(b, n, p);
@@ -2074,6 +2567,11 @@ class DeviceImpl final : public Device {
         }
         double *dst = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
         HIPCHK(hipMemcpy(dst + b, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        k_pack_pos<<<blocks(N_), kBlock, 0, stream_>>>(x_[cur_].p, y_[cur_].p, z_[cur_].p, p4_.p,
+                                                       pd_.p,
+                                                       N_);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
         list_valid_ = false;
         return SPH_OK;
       }
@@ -2087,6 +2585,7 @@ class DeviceImpl final : public Device {
     }
     if (name == "type") {
       list_valid_ = false;
+      verlet_valid_ = false;
       std::vector<int> tmp(n);
       for (int p = 0; p < n; ++p) {
         const int l = local_id(b, n, p);
@@ -2112,6 +2611,13 @@ class DeviceImpl final : public Device {
     if (!have_cells_) {
       err = "no neighbours_search has run";
       return SPH_ERR_STATE;
+    }
+    if (verlet_dirty_) {
+      // Steps with Verlet lists reorder only at rebuilds: the cell list of the
+      // current positions needs a full neighbour search.
+      int rc = settle(err);
+      if (rc) return rc;
+      HIPCHK(hipStreamSynchronize(stream_));
     }
     HIPCHK(hipStreamSynchronize(stream_));
     const SimInfo &si = sims_[sim];
@@ -2236,9 +2742,26 @@ class DeviceImpl final : public Device {
     for (int a = 0; a < 3; ++a) S.v[a] = v_[b][a].p;
     for (int a = 0; a < 6; ++a) S.c[a] = c_[b][a].p;
     S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
+    S.p4 = p4_.p;
+    S.pd = pd_.p;
     return S;
   }
   NList nlist() { return NList{nl_idx_.p, nl_count_.p, nl_queue_.p, nl_qlen_.p, kmax_}; }
+  // The Verlet lists while a Verlet step runs; no lists otherwise.
+  VList vlist() {
+    VList v{nullptr, nullptr, nullptr, 0, skin_frac_};
+    if (in_verlet_) {
+      v.idx = vl_idx_.p;
+      v.count = vl_count_.p;
+      v.mask = vl_mask_.p;
+      v.cap = vcap_;
+    }
+    return v;
+  }
+  VerletCtl vctl() {
+    VerletCtl v{vrebuild_.p, voverflow_.p, ref4_.p, 0.49f * skin_frac_, in_verlet_ ? 1 : 0};
+    return v;
+  }
 
   Derived<R> derived() {
     Derived<R> D;
@@ -2302,6 +2825,15 @@ class DeviceImpl final : public Device {
   bool any_walls_ = false;
   int nc_ = 3, cur_ = 0;
   int block_ = kBlock, xcd_ = 1, nblk_ = 0, max_parts_ = 1;
+  // Verlet lists (used by run_steps): enabled when skin > 0 and every periodic
+  // axis of every simulation has at least 5 cells.
+  bool verlet_ok_ = false, verlet_valid_ = false, verlet_dirty_ = false, in_verlet_ = false;
+  float skin_frac_ = 0.05f;
+  int vcap_ = 0;
+  DevBuf<int4> vl_idx_;
+  DevBuf<unsigned> vl_mask_;
+  DevBuf<int> vl_count_, vrebuild_, voverflow_;
+  DevBuf<float4> ref4_;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
   hipStream_t stream_ = nullptr;
   hipGraphExec_t exec_[2] = {nullptr, nullptr};
@@ -2311,6 +2843,10 @@ class DeviceImpl final : public Device {
   DevBuf<double> x_[2], y_[2], z_[2];
   DevBuf<R> v_[2][3], c_[2][6], mass_[2];
   DevBuf<int> type_[2], id_[2];
+  DevBuf<float4> p4_;
+  DevBuf<double4> pd_;
+  DevBuf<int> newpos_;  // old sorted position -> new, of the last stand-alone reorder
+  DevBuf<R> ptmp_;
   DevBuf<R> dens_, press_, dinv_, f_[3], dcdt_[6], gv_[9];
   DevBuf<PairRec<R>> rec_;
   bool rec_valid_ = false;

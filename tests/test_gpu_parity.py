@@ -181,12 +181,20 @@ def test_bitwise_repeatable(precision, tmp_path):
 
 def test_method_sequence_matches_step(tmp_path):
     """VV1, neighbours_search, set_step, calculate_forces, VV2 called one by one
-    give the same state as sph_step, whose steps replay captured graphs (one per
-    buffer parity); checked after every replay, with host reads in between."""
+    against sph_step, whose steps replay captured graphs (one per buffer parity)
+    and use Verlet lists (same pairs, list order instead of cell order):
+    FP64 agreement to 1e-12 after every replay, with host reads in between."""
     a = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="a")
     b = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="b")
     a.prepare()
     b.prepare()
+
+    def same(step):
+        for n in ["x", "y"]:
+            assert np.abs(a.get(n) - b.get(n)).max() <= 1e-13, f"{n} at step {step}"
+        for n in ["vx", "vy", "cxy", "fx", "dens", "coll_x", "tz_colloid"]:
+            assert relerr(a.get(n), b.get(n)) <= 1e-12, f"{n} at step {step}"
+
     for step in (1, 2, 3, 4, 5):
         a.V_Verlet_step1()
         a.neighbours_search()
@@ -195,18 +203,71 @@ def test_method_sequence_matches_step(tmp_path):
         a.V_Verlet_step2()
         b.step(1)
         b.synchronize()
-        for n in ["x", "y", "vx", "vy", "cxy", "fx", "dens", "coll_x", "tz_colloid"]:
-            np.testing.assert_array_equal(a.get(n), b.get(n), err_msg=f"{n} at step {step}")
-    # several steps in one call
-    for step in (6, 7, 8):
+        same(step)
+    for step in (6, 7, 8):   # several steps in one call
         a.V_Verlet_step1()
         a.neighbours_search()
         a.set_step(step)
         a.calculate_forces()
         a.V_Verlet_step2()
     b.step(3)
-    for n in ["x", "y", "vx", "vy", "cxy", "fx", "coll_x", "tz_colloid"]:
-        np.testing.assert_array_equal(a.get(n), b.get(n), err_msg=n)
+    same(8)
+
+
+@pytest.mark.parametrize("skin", ["0.005", "0.1"])
+@pytest.mark.parametrize("case", ["couette2d", "colloids3d"])
+def test_verlet_steps_fp64(case, skin, tmp_path, monkeypatch):
+    """40 steps with Verlet lists (a skin of 0.005 rcut rebuilds every few
+    steps, 0.1 rcut rarely) against the oracle, which re-sorts every step."""
+    monkeypatch.setenv("SPH_VERLET_SKIN", skin)
+    cfg = {"couette2d": lambda: helpers.cfg_couette(n=48, Vwall_bottom=-2.0, Vwall_top=2.0),
+           "colloids3d": lambda: helpers.cfg_colloids_3d(n=24)}[case]()
+    s = helpers.make_system(cfg, str(tmp_path), noise=1e-2)
+    helpers.perturb(s)
+    o = oracle.Oracle.from_system(s)
+    s.prepare()
+    o.prepare()
+    s.step(40)
+    o.step(40)
+    s.synchronize()
+    p = s.params
+    d = p.dim
+    for a, n in enumerate(comps("x", d)):
+        diff = helpers.periodic_diff(s.get(n), o.get(n), p.L[a])
+        assert np.max(np.abs(diff)) <= 1e-12, n
+    for n in comps("v", d) + comps("c", d) + comps("f", d) + ["dens"]:
+        assert relerr(s.get(n), o.get(n)) <= 1e-10, n
+    _check_cells(s, o)     # the cell list of the current positions
+
+
+def test_verlet_list_overflow(tmp_path, monkeypatch):
+    """Verlet lists of capacity 8 overflow everywhere: those particles fall back
+    to the cell scan, and their simulation is rebuilt every step."""
+    monkeypatch.setenv("SPH_VERLET_CAPACITY", "8")
+    s, o = _pair("colloids3d", str(tmp_path))
+    s.prepare()
+    o.prepare()
+    s.step(6)
+    o.step(6)
+    s.synchronize()
+    d = s.params.dim
+    for n in comps("v", d) + comps("f", d) + ["dens"]:
+        assert relerr(s.get(n), o.get(n)) <= 1e-10, n
+
+
+def test_neighbour_search_keeps_computed_fields(tmp_path):
+    """A stand-alone neighbours_search after particles moved reorders the state;
+    fields computed before it (forces, density, dc/dt) still belong to their
+    particles (the reference never permutes its arrays)."""
+    s, _ = _pair("sine3d", str(tmp_path))
+    s.prepare()
+    before = {n: s.get(n) for n in ("fx", "dens", "dcdt_xy", "press")}
+    x = s.get("x")
+    x = np.mod(x + 0.37, 1.0)          # move every particle across cells
+    s.set("x", x)
+    s.neighbours_search()
+    for n, v in before.items():
+        np.testing.assert_array_equal(s.get(n), v, err_msg=n)
 
 
 def test_individual_methods_fp64(tmp_path):
@@ -265,10 +326,13 @@ def test_out_of_grid_particle_is_reported(tmp_path):
 
 
 def test_small_neighbour_list_capacity_gives_identical_results(tmp_path, monkeypatch):
-    """With a list capacity of 8 every particle overflows; the pair pass then
-    re-scans its strips for the remaining neighbours in the same order, so the
-    results are bitwise those of the default capacity."""
+    """With a list capacity of 8 every particle overflows: the density pass adds
+    the hits past the capacity during its scan and the pair pass re-scans the
+    strips for them in the same order.  The pair sums are then bitwise those of
+    the default capacity; the density differs only by the association of its
+    sum (FP64: <= 1e-12 normwise after two steps, the parity bar)."""
     outs = []
+    names = ("dens", "fx", "fy", "fz", "dcdt_xy", "vx", "tz_colloid")
     for cap in (None, "8"):
         if cap:
             monkeypatch.setenv("SPH_NEIGHBOUR_CAPACITY", cap)
@@ -277,10 +341,10 @@ def test_small_neighbour_list_capacity_gives_identical_results(tmp_path, monkeyp
         s.prepare()
         s.step(2)
         s.synchronize()
-        outs.append([s.get(n) for n in ("fx", "fy", "fz", "dcdt_xy", "vx", "tz_colloid")])
+        outs.append([s.get(n) for n in names])
         s.close()
-    for a, b in zip(*outs):
-        np.testing.assert_array_equal(a, b)
+    for n, a, b in zip(names, *outs):
+        assert relerr(a, b) <= 1e-12, n
 
 
 @pytest.mark.parametrize("case", ["colloids3d", "colloids2d", "sine3d"])
