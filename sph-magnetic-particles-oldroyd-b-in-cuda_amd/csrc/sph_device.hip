@@ -995,24 +995,40 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
-  const double rv = P.rcut * (1.0 + (double)VL.skin_frac), rv2 = rv * rv;
+  const double rv = P.rcut * (1.0 + (double)VL.skin_frac);
+  const R rv2 = (R)(rv * rv);
   int4 *__restrict__ out = VL.idx + i;
   const int cap = VL.cap;
   int cnt = 0;
   int4 buf = make_int4(0, 0, 0, 0);
+  // The cut-off test of the density pass (float positions in the FP32 form),
+  // four candidates' loads in flight at a time.
+  CandSelf<R, DIM> me;
+  me.init(S, i);
+  auto test = [&](int k, const Cand<R, DIM> &c) {
+    if ((R)me.r2(c) < rv2 && k != i) {
+      buf = make_int4(buf.y, buf.z, buf.w, k);
+      cnt++;
+      if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
+    }
+  };
   for_each_strip5<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
                                                      double sz) {
+    me.shift(sx, sy, sz);
     int k = k0;
 #pragma unroll 1
+    for (; k + 4 <= k1; k += 4) {
+      Cand<R, DIM> c[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u].load(S, k + u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) test(k + u, c[u]);
+    }
+#pragma unroll 1
     for (; k < k1; ++k) {
-      double r[3];
-      seg_sep<double, DIM>(xi, yi, zi, S.x[k], S.y[k], DIM == 3 ? S.z[k] : 0.0, sx, sy, sz, r);
-      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-      if (r2 < rv2 && k != i) {
-        buf = make_int4(buf.y, buf.z, buf.w, k);
-        cnt++;
-        if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
-      }
+      Cand<R, DIM> c;
+      c.load(S, k);
+      test(k, c);
     }
   });
   const int tail = cnt & 3;

@@ -204,21 +204,26 @@ __global__ __launch_bounds__(256) void k_loss(const float *y, const float *t, co
   if (threadIdx.x == 0) loss[e] = SE * inv;
 }
 
-// db[e * sdb + n] = sum_b dZ[e][b][n] (bf16 in, fp32 out).  Block (x, e) covers
-// 64 columns: 4 row groups of 64 threads sum rows r, r + 4, ... and the groups
-// are added in a fixed order.
-__global__ __launch_bounds__(256) void k_colsum(const bf16_t *dz, float *db, long sdb, int B,
-                                                int N) {
-  __shared__ float part[4][64];
+// db[e * sdb + n] = sum_b dZ[e][b][n] (bf16 in, fp32 out).  Block (x, e) of
+// 1024 threads covers 64 columns: 16 row groups of 64 threads sum rows r,
+// r + 16, ... (independent loads, unrolled) and the groups are added in a
+// fixed order.
+__global__ __launch_bounds__(1024) void k_colsum(const bf16_t *dz, float *db, long sdb, int B,
+                                                 int N) {
+  __shared__ float part[16][64];
   const int e = blockIdx.y, c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
   float s = 0.f;
-  if (c < N)
-    for (int b = g; b < B; b += 4) s += bf2f(dz[((long)e * B + b) * N + c]);
+  if (c < N) {
+#pragma unroll 4
+    for (int b = g; b < B; b += 16) s += bf2f(dz[((long)e * B + b) * N + c]);
+  }
   part[g][threadIdx.x & 63] = s;
   __syncthreads();
-  if (g == 0 && c < N)
-    db[e * sdb + c] = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
-                      (part[2][threadIdx.x] + part[3][threadIdx.x]);
+  if (g == 0 && c < N) {
+    float t = 0.f;
+    for (int k = 0; k < 16; ++k) t += part[k][threadIdx.x];
+    db[e * sdb + c] = t;
+  }
 }
 
 // Adam on the fp32 masters; refreshes the bf16 copies (weights only).
@@ -403,7 +408,7 @@ int backward(sph_mlp_handle h) {
     a.sZ = L.per_member;
     int rc = gemm<true, false, E_STORE>(h, a);
     if (rc) return rc;
-    k_colsum<<<dim3((L.rows[l] + 63) / 64, E), 256, 0, h->stream>>>(
+    k_colsum<<<dim3((L.rows[l] + 63) / 64, E), 1024, 0, h->stream>>>(
         h->dz[l].p, h->g.p + L.b[l], L.per_member, B, L.rows[l]);
     MLPCHK(hipGetLastError());
     if (l == 0) break;

@@ -130,12 +130,16 @@ class Ensemble:
                                                _p(loss)), "mlp_gradients")
         return g, loss
 
-    def train_step(self, x, t, mask, lr):
+    def train_step(self, x, t, mask, lr, want_loss=True):
+        """One Adam step; returns the members' losses before it (None when
+        want_loss is False: the step is then queued without waiting)."""
         x, t, mask = _f32(x), _f32(t), _f32(mask)
         B = x.shape[1]
-        loss = np.zeros(self.E, np.float32)
+        loss = np.zeros(self.E, np.float32) if want_loss else None
         self._check(self.lib.sph_mlp_train_step(self.h, B, _p(x), _p(t), _p(mask),
-                                                ctypes.c_float(lr), _p(loss)), "mlp_train_step")
+                                                ctypes.c_float(lr),
+                                                _p(loss) if want_loss else None),
+                    "mlp_train_step")
         return loss
 
 
@@ -193,10 +197,12 @@ class Surrogate:
             rate = lr if lr_end is None else (
                 lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * s / steps)))
             idx = np.stack([b[rng.integers(0, n, batch)] for b in boot])
-            loss = self.net.train_step(Xs[idx], Ys[idx], M[idx], rate)
-            hist.append(float(loss.mean()))
-            if log and log_every and (s % log_every == 0 or s == steps - 1):
-                log(f"step {s}: loss {hist[-1]:.4e}")
+            probe = s % 10 == 0 or s == steps - 1     # read the loss back every 10 steps
+            loss = self.net.train_step(Xs[idx], Ys[idx], M[idx], rate, want_loss=probe)
+            if probe:
+                hist.append(float(loss.mean()))
+                if log and log_every and (s % log_every == 0 or s == steps - 1):
+                    log(f"step {s}: loss {hist[-1]:.4e}")
         return np.array(hist)
 
     def predict(self, X):
