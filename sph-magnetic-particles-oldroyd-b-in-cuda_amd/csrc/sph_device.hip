@@ -37,12 +37,20 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "sph_device.h"
+
+// Multiply-add contraction only inside one source expression (a * b + c in a
+// single statement becomes one fma), never across statements: the arithmetic
+// of a pair is then fixed by the source alone, so the Verlet walk, the strip
+// walk and the overflow tail - separate inlined copies of the same pair code -
+// round identically, and a batch reproduces its simulations run alone.
+#pragma clang fp contract(on)
 
 namespace sph {
 namespace {
@@ -174,6 +182,28 @@ __device__ __forceinline__ double mimg(double d, double L, double h) {
   return d;
 }
 
+// The periodic box of the pair loops: lengths and their reciprocals.
+struct Box {
+  double L[3], iL[3];
+};
+__device__ __forceinline__ Box box_of(const sph_params_t &P) {
+  Box b;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b.L[a] = P.L[a];
+    b.iL[a] = 1.0 / P.L[a];
+  }
+  return b;
+}
+
+// Minimum image as d - L rint(d / L) (three DP operations, no compares).  For
+// every separation that can be within reach (|d| < L/2 up to an ulp) it equals
+// mimg(d, L, L/2) bit for bit: the shift is 0 or +-1, so fma(-L, shift, d)
+// rounds d -+ L exactly once, as the subtraction does.
+__device__ __forceinline__ double mimg_r(double d, double L, double iL) {
+  return fma(-L, rint(d * iL), d);
+}
+
 // Square root and reciprocal of pair separations.  Double: correctly rounded,
 // like the reference's sqrt and division.  Float (the FP32 form): the hardware
 // v_sqrt_f32 / v_rcp_f32 (1 ulp) unless SPH_FP32_IEEE_MATH is defined.
@@ -201,9 +231,12 @@ __device__ __forceinline__ int xcd_tile() {
 }
 
 // Minimum resident wavefronts per SIMD requested for the fused pair kernel
-// (caps its registers).
-#ifndef SPH_PAIRS_MIN_WAVES
-#define SPH_PAIRS_MIN_WAVES 1
+// (caps its registers), per precision form.
+#ifndef SPH_PAIRS_WAVES_F32
+#define SPH_PAIRS_WAVES_F32 1
+#endif
+#ifndef SPH_PAIRS_WAVES_F64
+#define SPH_PAIRS_WAVES_F64 1
 #endif
 
 // kernel_W (src/kernel_W.cu:13-59) as selects: cw*((a^5 - 6 b^5) + 15 c^5).
@@ -414,23 +447,37 @@ struct CandSelf<float, DIM> {
 // r_ij = x_i - x_j with the minimum image (x, z, and y without walls), computed
 // in double and rounded to R.
 template <class R, int DIM, bool WALL>
-__device__ __forceinline__ void pair_sep(const sph_params_t &P, double xi, double yi, double zi,
+__device__ __forceinline__ void pair_sep(const Box &X, double xi, double yi, double zi,
                                          double xj, double yj, double zj, R r[3]) {
-  r[0] = (R)mimg(xi - xj, P.L[0], 0.5 * P.L[0]);
-  r[1] = WALL ? (R)(yi - yj) : (R)mimg(yi - yj, P.L[1], 0.5 * P.L[1]);
-  r[2] = DIM == 3 ? (R)mimg(zi - zj, P.L[2], 0.5 * P.L[2]) : R(0);
+  r[0] = (R)mimg_r(xi - xj, X.L[0], X.iL[0]);
+  r[1] = WALL ? (R)(yi - yj) : (R)mimg_r(yi - yj, X.L[1], X.iL[1]);
+  r[2] = DIM == 3 ? (R)mimg_r(zi - zj, X.L[2], X.iL[2]) : R(0);
 }
+
+// The parameters the pair body reads, copied out of the parameter block once
+// per thread; read through the parameter pointer inside the neighbour loop
+// they were re-loaded for every neighbour.
+template <class R>
+struct PairK {
+  Box X;
+  double y_bottom, y_top;
+  R rcut_inv, c_gradw, bmax, nkT, a, b, V_bottom, V_top, coll_R;
+  __device__ explicit PairK(const sph_params_t &P)
+      : X(box_of(P)), y_bottom(P.y_bottom), y_top(P.y_top), rcut_inv((R)P.rcut_inv),
+        c_gradw((R)P.c_gradw), bmax((R)P.beta_max), nkT((R)P.nkT), a((R)P.a), b((R)P.b),
+        V_bottom((R)P.V_bottom), V_top((R)P.V_top), coll_R((R)P.coll_R) {}
+};
 
 // Relative velocity v_ij with Morris boundary conditions for wall and colloid
 // partners (src/kernel_vel_gradients.cu:157-332, 415-630; the same branches in
 // src/kernel_forces.cu:196-419, 533-822).  Called only when ti == tj == 0 or
 // ti != tj.
 template <class R, int DIM, bool WALL>
-__device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &B, int ti, int tj,
+__device__ __forceinline__ void morris_vij(const PairK<R> &K, const Bodies &B, int ti, int tj,
                                            double xi, double yi, double zi, double xj, double yj,
                                            double zj, const R vi[3], const R vj[3],
                                            const R rij[3], R v[3]) {
-  const R bmax = (R)P.beta_max;
+  const R bmax = K.bmax;
   if (ti == tj) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = vi[a] - vj[a];
@@ -440,9 +487,9 @@ __device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &
     if (tj == 1 || tj == 2) {
       R di, dj, Vw;
       if (tj == 1) {
-        di = (R)(yi - P.y_bottom); dj = (R)(P.y_bottom - yj); Vw = (R)P.V_bottom;
+        di = (R)(yi - K.y_bottom); dj = (R)(K.y_bottom - yj); Vw = K.V_bottom;
       } else {
-        di = (R)(P.y_top - yi); dj = (R)(yj - P.y_top); Vw = (R)P.V_top;
+        di = (R)(K.y_top - yi); dj = (R)(yj - K.y_top); Vw = K.V_top;
       }
       R beta = R(1) + dj / di;
       if (beta > bmax) beta = bmax;
@@ -454,10 +501,10 @@ __device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &
     } else {
       const int c = tj - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(P, xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
+      pair_sep<R, DIM, WALL>(K.X, xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
                              DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
-      const R Rc = (R)P.coll_R;
+      const R Rc = K.coll_R;
       R dis = dc - Rc;
       const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
       if (dis < R(0)) {
@@ -480,9 +527,9 @@ __device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &
     if (ti == 1 || ti == 2) {
       R di, dj, Vw;
       if (ti == 1) {
-        di = (R)(P.y_bottom - yi); dj = (R)(yj - P.y_bottom); Vw = (R)P.V_bottom;
+        di = (R)(K.y_bottom - yi); dj = (R)(yj - K.y_bottom); Vw = K.V_bottom;
       } else {
-        di = (R)(yi - P.y_top); dj = (R)(P.y_top - yj); Vw = (R)P.V_top;
+        di = (R)(yi - K.y_top); dj = (R)(K.y_top - yj); Vw = K.V_top;
       }
       R beta = R(1) + di / dj;
       if (beta > bmax) beta = bmax;
@@ -494,10 +541,10 @@ __device__ __forceinline__ void morris_vij(const sph_params_t &P, const Bodies &
     } else {
       const int c = ti - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(P, xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
+      pair_sep<R, DIM, WALL>(K.X, xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
                              DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
-      const R Rc = (R)P.coll_R;
+      const R Rc = K.coll_R;
       R djs = dc - Rc;
       const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
       if (djs < R(0)) {
@@ -801,6 +848,56 @@ struct alignas(128) PairRec<double> {
 static_assert(sizeof(PairRec<float>) == 64, "FP32 pair record must be one 64-byte line");
 static_assert(sizeof(PairRec<double>) == 128, "FP64 pair record must be two lines");
 
+// 16-byte vectors holding the used part of a record (the FP64 record's padding
+// is skipped).
+template <class R>
+constexpr int kRecVecs = (int)((offsetof(PairRec<R>, q) + 6 * sizeof(R) + 15) / 16);
+
+// The record of particle j (global index).
+template <class R>
+__device__ __forceinline__ PairRec<R> load_rec(const PairRec<R> *__restrict__ rec, int j) {
+  const float4 *__restrict__ p = reinterpret_cast<const float4 *>(rec + j);
+  float4 v[kRecVecs<R>];
+#pragma unroll
+  for (int k = 0; k < kRecVecs<R>; ++k) v[k] = p[k];
+  PairRec<R> r;
+  __builtin_memcpy(&r, v, sizeof(v));
+  return r;
+}
+
+// The records of one simulation as a buffer resource (a scalar descriptor:
+// base, size in bytes, raw addressing; 0x00020000 is the gfx9 word 3).  Record
+// loads through it are whole dwordx4 buffer loads - a record costs four L1
+// accesses per lane (FP32; seven FP64) where field-wise global loads cost five -
+// and an offset past the simulation's records reads zeros instead of faulting.
+// Offsets are 32-bit: the host limits a simulation to 2^32 - 1 bytes of records.
+struct RecBuf {
+  __amdgpu_buffer_rsrc_t rs;
+};
+template <class R>
+__device__ __forceinline__ RecBuf rec_buf(const PairRec<R> *rec, const SimTable &T, int sim) {
+  const int b = T.pbeg[sim], n = T.pbeg[sim + 1] - b;
+  RecBuf r;
+  r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<PairRec<R> *>(rec + b), 0,
+                                           (int)((unsigned)n * (unsigned)sizeof(PairRec<R>)),
+                                           0x00020000);
+  return r;
+}
+
+// The record of particle jl (index within the simulation) through the buffer.
+template <class R>
+__device__ __forceinline__ PairRec<R> load_rec(const RecBuf &rb, int jl) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  const unsigned off = (unsigned)jl * (unsigned)sizeof(PairRec<R>);
+  u4 v[kRecVecs<R>];
+#pragma unroll
+  for (int k = 0; k < kRecVecs<R>; ++k)
+    v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, (int)(off + 16u * k), 0, 0);
+  PairRec<R> r;
+  __builtin_memcpy(&r, v, sizeof(v));
+  return r;
+}
+
 // Pressure p = c^2 (m d - rho0) (src/kernel_pressures.cu:21), 1/d, and the pair
 // record of particle i, with Pi/d^2 in the reference's arithmetic
 // (src/kernel_forces.cu:64-88, 547-558).
@@ -859,6 +956,7 @@ __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, 
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
   const R rcutsq = (R)P.rcutsq, rinv = (R)P.rcut_inv, cw = (R)P.cw;
+  const Box bx = box_of(P);
   const int kmax = L.kmax;
   int cnt = 0;
   R d = R(0);
@@ -882,7 +980,7 @@ __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, 
       for (int u = 0; u < 4; ++u) {
         if (u >= m) break;
         R r[3];
-        pair_sep<R, DIM, WALL>(P, xi, yi, zi, pj[u].x, pj[u].y, DIM == 3 ? pj[u].z : 0.0, r);
+        pair_sep<R, DIM, WALL>(bx, xi, yi, zi, pj[u].x, pj[u].y, DIM == 3 ? pj[u].z : 0.0, r);
         const R r2 =
             DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
         if (r2 < rcutsq) {
@@ -959,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, 
         for (int u = 0; u < 4; ++u) {
           if (u >= m) break;
           R r[3];
-          pair_sep<R, DIM, WALL>(P, xi, yi, zi, xj[u], yj[u], zj[u], r);
+          pair_sep<R, DIM, WALL>(bx, xi, yi, zi, xj[u], yj[u], zj[u], r);
           const R r2 =
               DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
           d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
@@ -1126,24 +1224,23 @@ struct PairSelf {
 // reference's pair selection, Morris velocities and Pi/d^2 substitutions
 // (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__device__ __forceinline__ void pair_body(const sph_params_t &P, const PairRec<R> *__restrict__ rec,
-                                          const Bodies &B, const PairSelf<R> &me, int j,
+__device__ __forceinline__ void pair_body(const PairK<R> &K, const PairRec<R> &rj,
+                                          const Bodies &B, const PairSelf<R> &me,
                                           PairAcc<R> &acc) {
-  const PairRec<R> rj = rec[j];
   const int tj = rj.w < R(0) ? (int)(-rj.w) : 0;
   const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
   const bool on_f = DO_F && (me.t == 0 || tj == 0);
   if (!on_gv && !on_f) return;
   const double xj = rj.x, yj = rj.y, zj = rj.z;
   R r[3];
-  pair_sep<R, DIM, WALL>(P, me.x, me.y, me.z, xj, yj, zj, r);
+  pair_sep<R, DIM, WALL>(K.X, me.x, me.y, me.z, xj, yj, zj, r);
   const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
   const R rr = pair_sqrt(r2), ri = pair_rcp(rr);
   const R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
-  const R gw = sph_gradW<R>(rr, (R)P.rcut_inv, (R)P.c_gradw);
+  const R gw = sph_gradW<R>(rr, K.rcut_inv, K.c_gradw);
   const R vj[3] = {rj.v[0], rj.v[1], rj.v[2]};
   R v[3];
-  morris_vij<R, DIM, WALL>(P, B, me.t, tj, me.x, me.y, me.z, xj, yj, zj, me.v, vj, r, v);
+  morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, xj, yj, zj, me.v, vj, r, v);
   if (on_gv) {
     // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
     const R coef = -me.dinv * gw;
@@ -1167,7 +1264,7 @@ __device__ __forceinline__ void pair_body(const sph_params_t &P, const PairRec<R
     } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure, keeps its c
       diinv = me.dinv;
       djinv = me.dinv;
-      const R nkT = (R)P.nkT, d2 = djinv * djinv;
+      const R nkT = K.nkT, d2 = djinv * djinv;
 #pragma unroll
       for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
       Pj[0] = (me.p - nkT * rj.q[0]) * d2;
@@ -1196,7 +1293,7 @@ __device__ __forceinline__ void pair_body(const sph_params_t &P, const PairRec<R
       acc.f[1] += axy * e[0] + ayy * e[1];
     }
     const R aux = gw * diinv * djinv * ri;
-    const R a1 = (R)P.a * aux, a2 = (R)P.b * aux * ev;
+    const R a1 = K.a * aux, a2 = K.b * aux * ev;
     acc.f[0] += a1 * v[0] + a2 * e[0];
     acc.f[1] += a1 * v[1] + a2 * e[1];
     if (DIM == 3) acc.f[2] += a1 * v[2] + a2 * e[2];
@@ -1288,7 +1385,7 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // DO_GV / DO_F select the parts.  A particle with more neighbours than the list
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_PAIRS_WAVES_F64) void k_pairs(
     SimTable T, Particles<R> S, Derived<R> D, Bodies Ball, const PairRec<R> *__restrict__ rec,
     NList L, int store_gv, int write_dcdt, int with_ext, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
@@ -1306,21 +1403,63 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
 #pragma unroll
     for (int b = 0; b < 3; ++b) acc.g[a][b] = R(0);
   }
+  const PairK<R> K(P);
   if (VL.idx && VL.count[i] <= VL.cap) {
-    // the Verlet entries the density pass marked, in list order; all lanes walk
-    // the list groups together so every index load is coalesced
+    // The Verlet entries the density pass marked, in list order; all lanes
+    // step through the entries together, so index and mask loads are
+    // coalesced.
     const int vc = VL.count[i];
-    unsigned m = 0u;
+    const int4 *__restrict__ vl = VL.idx + i;
+    const unsigned *__restrict__ vm = VL.mask + i;
+    const int pb = T.pbeg[sim];
+    const RecBuf rb = rec_buf(rec, T, sim);
+    if constexpr (sizeof(R) == 4) {
+      // FP32 form: the record of entry e + 1 is loaded before entry e is
+      // evaluated, so a gather's latency overlaps the previous pair's
+      // arithmetic.  Every load of the loop is unconditional - a lane whose
+      // next entry is not marked (or past its list) loads its own record, and
+      // index and mask loads past the list re-load its last group - so the
+      // compiler's wait before a pair's arithmetic covers only the older loads.
+      // (The FP64 form's double-buffered records would halve its occupancy.)
+      const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
+      // q: indices of the current group of four, rotated so that q.x is the
+      // next entry; qn: the following group.  m: hit bits of the current 32
+      // entries, bit 0 the next entry; mn: the following 32.
+      int4 q = vl[0];
+      int4 qn = vl[(size_t)min(1, lastg) * N];
+      unsigned m = vm[0];
+      unsigned mn = vm[(size_t)min(1, lastw) * N];
+      PairRec<R> cur = load_rec<R>(rb, ((vc > 0 && (m & 1u)) ? q.x : i) - pb);
 #pragma unroll 1
-    for (int g = 0; g < vc; g += 4) {
-      if ((g & 31) == 0) m = VL.mask[(size_t)(g >> 5) * N + i];
-      int4 q = VL.idx[(size_t)(g >> 2) * N + i];
-      unsigned nib = (m >> (g & 31)) & 0xFu;
+      for (int e = 0; e < vc; ++e) {
+        const bool hit = m & 1u;
+        m >>= 1;
+        q = make_int4(q.y, q.z, q.w, q.x);
+        if (((e + 1) & 3) == 0) {
+          q = qn;
+          qn = vl[(size_t)min((e + 5) >> 2, lastg) * N];
+        }
+        if (((e + 1) & 31) == 0) {
+          m = mn;
+          mn = vm[(size_t)min((e + 33) >> 5, lastw) * N];
+        }
+        const PairRec<R> nxt = load_rec<R>(rb, ((e + 1 < vc && (m & 1u)) ? q.x : i) - pb);
+        if (hit) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, cur, B, me, acc);
+        cur = nxt;
+      }
+    } else {
+      unsigned m = 0u;
 #pragma unroll 1
-      for (int u = 0; u < 4; ++u) {
-        if (nib & 1u) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, q.x, acc);
-        nib >>= 1;
-        q = make_int4(q.y, q.z, q.w, 0);
+      for (int g = 0; g < vc; g += 4) {
+        if ((g & 31) == 0) m = vm[(size_t)(g >> 5) * N];
+        int4 q = vl[(size_t)(g >> 2) * N];
+        unsigned nib = (m >> (g & 31)) & 0xFu;
+#pragma unroll 1
+        for (int u = 0; u < 4; ++u) {
+          if (nib & 1u) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R>(rb, q.x - pb), B, me, acc);
+          nib >>= 1;
+          q = make_int4(q.y, q.z, q.w, 0);
+        }
       }
     }
     pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
@@ -1329,13 +1468,15 @@ __global__ __launch_bounds__(kBlock, SPH_PAIRS_MIN_WAVES) void k_pairs(
   const int total = L.count[i];
   const int cnt = min(total, L.kmax);
   const int4 *__restrict__ nl = L.idx + i;
+  const int pbn = T.pbeg[sim];
+  const RecBuf rbn = rec_buf(rec, T, sim);
 #pragma unroll 1
   for (int k = 0; k < cnt; k += 4) {
     int4 q = nl[(size_t)(k >> 2) * N];
     const int m = min(4, cnt - k);
 #pragma unroll 1
     for (int u = 0; u < m; ++u) {
-      pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, q.x, acc);
+      pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R>(rbn, q.x - pbn), B, me, acc);
       q = make_int4(q.y, q.z, q.w, 0);
     }
   }
@@ -1379,6 +1520,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
       for (int b = 0; b < 3; ++b) acc.g[a][b] = D.gv[a * 3 + b][i];
     }
     const R rcutsq = (R)P.rcutsq;
+    const PairK<R> K(P);
     const int skip = L.kmax;
     int hit = 0;
     int cx, cy, cz;
@@ -1393,7 +1535,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
         c.load(S, k);
         const R r2 = (R)cme.r2(c);
         if (r2 < rcutsq && k != i) {
-          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(P, rec, B, me, k, acc);
+          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec(rec, k), B, me, acc);
           hit++;
         }
       }
@@ -1888,6 +2030,13 @@ class DeviceImpl final : public Device {
       const sph_params_t &p = h->p;
       if (p.dim != P_.dim || p.wall != P_.wall) {
         err = "all simulations of a batch must share dim and wall";
+        return SPH_ERR_ARG;
+      }
+      if ((unsigned long long)p.N * sizeof(PairRec<R>) > 0xFFFFFFFFull) {
+        // record loads address a simulation's records with 32-bit offsets
+        err = "a simulation of " + std::to_string(p.N) + " particles exceeds the " +
+              std::to_string(0xFFFFFFFFull / sizeof(PairRec<R>)) +
+              "-particle limit of this precision form";
         return SPH_ERR_ARG;
       }
       SimInfo si;
