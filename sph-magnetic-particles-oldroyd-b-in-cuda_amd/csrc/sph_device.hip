@@ -238,6 +238,10 @@ __device__ __forceinline__ int xcd_tile() {
 #ifndef SPH_PAIRS_WAVES_F64
 #define SPH_PAIRS_WAVES_F64 1
 #endif
+// and for the density pass
+#ifndef SPH_DENSITY_WAVES
+#define SPH_DENSITY_WAVES 1
+#endif
 
 // kernel_W (src/kernel_W.cu:13-59) as selects: cw*((a^5 - 6 b^5) + 15 c^5).
 template <class R>
@@ -870,6 +874,11 @@ template <int K>
 __device__ __forceinline__ int quad_bcast(int x) {
   return __builtin_amdgcn_update_dpp(0, x, K * 0x55, 0xF, 0xF, false);
 }
+// DPP quad permutation with control byte C (lane 4a + b reads lane 4a + ((C >> 2b) & 3)).
+template <int C>
+__device__ __forceinline__ int quad_pick(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, C, 0xF, 0xF, false);
+}
 
 // The records of one simulation as a buffer resource (a scalar descriptor:
 // base, size in bytes, raw addressing; 0x00020000 is the gfx9 word 3).  Record
@@ -882,7 +891,10 @@ struct RecBuf {
 };
 template <class R>
 __device__ __forceinline__ RecBuf rec_buf(const PairRec<R> *rec, const SimTable &T, int sim) {
-  const int b = T.pbeg[sim], n = T.pbeg[sim + 1] - b;
+  // uniform by construction (one simulation per workgroup); readfirstlane keeps
+  // the descriptor in scalar registers where the compiler cannot prove it
+  const int b = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
+  const int n = __builtin_amdgcn_readfirstlane(T.pbeg[sim + 1]) - b;
   RecBuf r;
   r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<PairRec<R> *>(rec + b), 0,
                                            (int)((unsigned)n * (unsigned)sizeof(PairRec<R>)),
@@ -944,16 +956,22 @@ __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Part
 }
 
 template <class R, int DIM, bool WALL>
-__global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, Derived<R> D,
+__global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable T, Particles<R> S, Derived<R> D,
                                                     PairRec<R> *rec, const int *__restrict__ csg,
                                                     NList L, int write_dens, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int *__restrict__ cs = csg + T.cell_off[sim];
   const int N = T.ntotal;  // list stride
-  const int i = T.blk_first[b] + threadIdx.x;
-  if (i == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
-  if (i >= T.pbeg[sim + 1]) return;
+  const int iend = T.pbeg[sim + 1];
+  const int i0 = T.blk_first[b] + threadIdx.x;
+  if (i0 == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
+  const bool live = i0 < iend;
+  // Verlet walk: every lane of a wave takes part in the pair loads, so the
+  // lanes past the simulation's end stay (aliasing its last particle for their
+  // reads) and store nothing.
+  if (!live && !VL.idx) return;
+  const int i = live ? i0 : iend - 1;
   const double *__restrict__ X = S.x;
   const double *__restrict__ Y = S.y;
   const double *__restrict__ Z = S.z;
@@ -966,41 +984,97 @@ __global__ __launch_bounds__(kBlock) void k_density(SimTable T, Particles<R> S, 
   const int kmax = L.kmax;
   int cnt = 0;
   R d = R(0);
-  if (VL.idx && VL.count[i] <= VL.cap) {
-    // Verlet list: the hits in list order, W summed as they come (most list
-    // entries are hits, so lanes stay converged); the hits are marked in the
-    // mask words, stored as each fills (all lanes store word w together).
-    d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
-    unsigned mw = 0u;
-    const int vc = VL.count[i];
-    const int4 *__restrict__ vl = VL.idx + i;
-#pragma unroll 1
-    for (int g = 0; g < vc; g += 4) {
-      const int4 q = vl[(size_t)(g >> 2) * N];
-      const int m = min(4, vc - g);
-      const int jj[4] = {q.x, q.y, q.z, q.w};
-      double4 pj[4];
+  const bool vok = VL.idx && live && VL.count[i] <= VL.cap;
+  if (VL.idx) {
+    // Verlet list: the hits in list order, W summed as they come; all lanes of
+    // the wave step through the list groups (four entries) together (a lane
+    // past its list or not on the Verlet path has no entries), so index loads
+    // are coalesced.  The hits are marked in the mask words, stored as each
+    // fills.  Positions (double4, 32 bytes) move in two stages: the two lanes
+    // of pair a load, in instruction k, the two 16-byte halves of the position
+    // lane 2a + k needs (one line per lane pair instead of one per lane), then
+    // the halves go through LDS to the lane that needs them.  The loads of
+    // group g + 1 are issued before group g is evaluated; a lane without an
+    // entry asks for an offset past the simulation's positions (zeros, no
+    // memory access).
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    constexpr int kSlot = 3;  // 48-byte slots: conflict-free LDS rows
+    __shared__ u4 stage[kBlock / 64][64][kSlot];
+    u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63, hb = lane & 1, pa = lane & ~1;
+    const int vc = vok ? VL.count[i] : 0;
+    int vmax = vc;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) pj[u] = S.pd[u < m ? jj[u] : i];
+    for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
+    vmax = __builtin_amdgcn_readfirstlane(vmax);
+    const int4 *__restrict__ vl = VL.idx + i;
+    // (uniform by construction; readfirstlane keeps the descriptor scalar)
+    const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
+    const int pe = __builtin_amdgcn_readfirstlane(iend);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double4 *>(S.pd + pb), 0, (int)((unsigned)(pe - pb) * 32u), 0x00020000);
+    const unsigned oob = (unsigned)(pe - pb) * 32u;
+    const int lastg = max(vc - 1, 0) >> 2;
+    auto group_load = [&](const int4 &g4, int e0, u4(&v)[4][2]) {
+      const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (u >= m) break;
-        R r[3];
-        pair_sep<R, DIM, WALL>(bx, xi, yi, zi, pj[u].x, pj[u].y, DIM == 3 ? pj[u].z : 0.0, r);
-        const R r2 =
-            DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-        if (r2 < rcutsq) {
-          d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
-          mw |= 1u << ((g + u) & 31);
-          cnt++;
+        // lane pa + k's position offset (broadcast in the pair) plus this lane's half
+        const int mine = (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 32u : oob), ho = 16 * hb;
+        v[u][0] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xA0>(mine) + ho, 0, 0);
+        v[u][1] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xF5>(mine) + ho, 0, 0);
+      }
+    };
+    // q: indices of the current group, qn: of the following group
+    int4 q = vl[0];
+    int4 qn = vl[(size_t)min(1, lastg) * N];
+    u4 cur[4][2];
+    group_load(q, 0, cur);
+    if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+    unsigned mw = 0u;
+#pragma unroll 1
+    for (int g = 0; g < vmax; g += 4) {
+      u4 nxt[4][2];
+      group_load(qn, g + 4, nxt);
+      q = qn;
+      qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        // half hb of the position of lane pa + k -> slot pa + k
+        st[pa][hb] = cur[u][0];
+        st[pa + 1][hb] = cur[u][1];
+        __builtin_amdgcn_wave_barrier();
+        u4 mine[2];
+        mine[0] = st[lane][0];
+        mine[1] = st[lane][1];
+        __builtin_amdgcn_wave_barrier();
+        if (g + u < vc) {
+          double4 pj;
+          __builtin_memcpy(&pj, mine, sizeof(pj));
+          R r[3];
+          pair_sep<R, DIM, WALL>(bx, xi, yi, zi, pj.x, pj.y, DIM == 3 ? pj.z : 0.0, r);
+          const R r2 =
+              DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+          if (r2 < rcutsq) {
+            d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+            mw |= 1u << ((g + u) & 31);
+            cnt++;
+          }
         }
       }
-      if (((g + 4) & 31) == 0 || g + 4 >= vc) {
+      if (g < vc && (((g + 4) & 31) == 0 || g + 4 >= vc)) {
         VL.mask[(size_t)(g >> 5) * N + i] = mw;
         mw = 0u;
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u][0] = nxt[u][0];
+        cur[u][1] = nxt[u][1];
+      }
     }
-  } else {
+    if (!live) return;
+  }
+  if (!vok) {
     R d_over = R(0);  // W of the hits past the list capacity, in scan order
     int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
     // Phase 1: the candidate test only.  Four candidates' positions are loaded
