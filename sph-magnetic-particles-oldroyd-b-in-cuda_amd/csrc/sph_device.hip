@@ -1472,11 +1472,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   const int iend = T.pbeg[sim + 1];
   const int i0 = T.blk_first[b] + threadIdx.x;
   const bool live = i0 < iend;
-  // FP32 Verlet walk: every lane of a wave takes part in the quad loads, so
-  // the lanes past the simulation's end stay (aliasing its last particle for
-  // their reads) and store nothing.
-  constexpr bool kCoop = sizeof(R) == 4;
-  if (!live && !(kCoop && VL.idx)) return;
+  // Verlet walk: every lane of a wave takes part in the quad loads, so the
+  // lanes past the simulation's end stay (aliasing its last particle for their
+  // reads) and store nothing.
+  if (!live && !VL.idx) return;
   const int i = live ? i0 : iend - 1;
   const sph_params_t &P = T.P[sim];
   const Bodies B = bodies_of(Ball, T, sim);
@@ -1492,116 +1491,98 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   }
   const PairK<R> K(P);
   const bool vok = VL.idx && live && VL.count[i] <= VL.cap;
-  if constexpr (kCoop) {
-    if (VL.idx) {
-      // The Verlet entries the density pass marked, in list order, all lanes of
-      // the wave stepping through entry e together (a lane past its list or not
-      // on the Verlet path has no hits).  Index and mask loads are coalesced.
-      // Records move in two stages: (1) the four lanes of quad a load, in
-      // instruction k, the four 16-byte quarters of the record lane 4a + k
-      // needs, so an instruction touches one line per quad instead of one per
-      // lane; (2) the quarters go through LDS to the lane that needs them.  The
-      // quad loads of entry e + 1 are issued before entry e is evaluated.  A
-      // lane without a hit asks for an offset past the simulation's records,
-      // which the buffer returns as zeros without a memory access.
-      typedef unsigned u4 __attribute__((ext_vector_type(4)));
-      constexpr int kSlot = 5;  // 80-byte record slots: conflict-free LDS rows
-      __shared__ u4 stage[kBlock / 64][64][kSlot];
-      u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
-      const int lane = threadIdx.x & 63, qb = lane & 3, qa = lane & ~3;
-      const int vc = vok ? VL.count[i] : 0;
-      int vmax = vc;
+  if (VL.idx) {
+    // The Verlet entries the density pass marked, in list order, all lanes of
+    // the wave stepping through entry e together (a lane past its list or not
+    // on the Verlet path has no hits).  Index and mask loads are coalesced.
+    // Records move in two stages: (1) the four lanes of quad a load the record
+    // lane 4a + k needs as 16-byte chunks, lane 4a + b taking chunk b (FP32:
+    // one instruction per record; FP64: two, chunks b and 4 + b), so an
+    // instruction touches one line per quad instead of one per lane; (2) the
+    // chunks go through LDS to the lane that needs them.  The quad loads of
+    // entry e + 1 are issued before entry e is evaluated.  A lane without a hit
+    // asks for an offset past the simulation's records, which the buffer
+    // returns as zeros without a memory access.
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    constexpr int kChunks = sizeof(PairRec<R>) / 16;  // 4 (FP32) or 8 (FP64, last is padding)
+    constexpr int kLoads = kChunks;                    // quad instructions per entry
+    constexpr int kSlot = kChunks + 1;  // padded LDS slots: conflict-free rows
+    __shared__ u4 stage[kBlock / 64][64][kSlot];
+    u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63, qb = lane & 3, qa = lane & ~3;
+    const int vc = vok ? VL.count[i] : 0;
+    int vmax = vc;
 #pragma unroll
-      for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
-      vmax = __builtin_amdgcn_readfirstlane(vmax);
-      const int4 *__restrict__ vl = VL.idx + i;
-      const unsigned *__restrict__ vm = VL.mask + i;
-      const int pb = T.pbeg[sim];
-      const RecBuf rb = rec_buf(rec, T, sim);
-      const unsigned oob = (unsigned)(iend - pb) * (unsigned)sizeof(PairRec<R>);
-      const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
-      // q: indices of the current group of four, rotated so that q.x is the
-      // next entry; qn: the following group.  m: hit bits of the current 32
-      // entries, bit 0 the next entry (masked to zero past the list); mn: the
-      // following 32 (unmasked until used).
-      int4 q = vl[0];
-      int4 qn = vl[(size_t)min(1, lastg) * N];
-      unsigned m = vm[0];
-      unsigned mn = vm[(size_t)min(1, lastw) * N];
-      if (vc < 32) m &= (1u << vc) - 1u;  // (vc = 0: no hits)
-      auto quad_load = [&](unsigned off, u4(&v)[4]) {
-        // lane qa + k's record offset (broadcast in the quad) plus this lane's quarter
-        const int mine = (int)off, qo = 16 * qb;
-        v[0] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, quad_bcast<0>(mine) + qo, 0, 0);
-        v[1] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, quad_bcast<1>(mine) + qo, 0, 0);
-        v[2] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, quad_bcast<2>(mine) + qo, 0, 0);
-        v[3] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, quad_bcast<3>(mine) + qo, 0, 0);
-      };
-      u4 cq[4];
-      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, cq);
-#pragma unroll 1
-      for (int e = 0; e < vmax; ++e) {
-        const bool hit = m & 1u;
-        m >>= 1;
-        q = make_int4(q.y, q.z, q.w, q.x);
-        if (((e + 1) & 3) == 0) {
-          q = qn;
-          qn = vl[(size_t)min((e + 5) >> 2, lastg) * N];
-        }
-        if (((e + 1) & 31) == 0) {
-          m = mn;
-          mn = vm[(size_t)min((e + 33) >> 5, lastw) * N];
-          const int left = vc - (e + 1);
-          if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
-        }
-        u4 nq[4];
-        quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, nq);
-        // quarter qb of the record of lane qa + k -> slot qa + k
-#pragma unroll
-        for (int k = 0; k < 4; ++k) st[qa + k][qb] = cq[k];
-        __builtin_amdgcn_wave_barrier();
-        u4 mine[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) mine[k] = st[lane][k];
-        __builtin_amdgcn_wave_barrier();
-        if (hit) {
-          PairRec<R> cur;
-          __builtin_memcpy(&cur, mine, sizeof(mine));
-          pair_body<R, DIM, WALL, DO_GV, DO_F>(K, cur, B, me, acc);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cq[k] = nq[k];
-      }
-      if (!live) return;
-      if (vok) {
-        pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
-        return;
-      }
-    }
-  } else if (vok) {
-    // The Verlet entries the density pass marked, in list order; all lanes
-    // step through the entries together, so index and mask loads are
-    // coalesced.  (Double-buffered FP64 records would halve the occupancy.)
-    const int vc = VL.count[i];
+    for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
+    vmax = __builtin_amdgcn_readfirstlane(vmax);
     const int4 *__restrict__ vl = VL.idx + i;
     const unsigned *__restrict__ vm = VL.mask + i;
-    const int pb = T.pbeg[sim];
     const RecBuf rb = rec_buf(rec, T, sim);
-    unsigned m = 0u;
-#pragma unroll 1
-    for (int g = 0; g < vc; g += 4) {
-      if ((g & 31) == 0) m = vm[(size_t)(g >> 5) * N];
-      int4 q = vl[(size_t)(g >> 2) * N];
-      unsigned nib = (m >> (g & 31)) & 0xFu;
-#pragma unroll 1
-      for (int u = 0; u < 4; ++u) {
-        if (nib & 1u) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R>(rb, q.x - pb), B, me, acc);
-        nib >>= 1;
-        q = make_int4(q.y, q.z, q.w, 0);
+    const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
+    const unsigned oob =
+        (unsigned)(__builtin_amdgcn_readfirstlane(iend) - pb) * (unsigned)sizeof(PairRec<R>);
+    const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
+    // q: indices of the current group of four, rotated so that q.x is the
+    // next entry; qn: the following group.  m: hit bits of the current 32
+    // entries, bit 0 the next entry (masked to zero past the list); mn: the
+    // following 32 (unmasked until used).
+    int4 q = vl[0];
+    int4 qn = vl[(size_t)min(1, lastg) * N];
+    unsigned m = vm[0];
+    unsigned mn = vm[(size_t)min(1, lastw) * N];
+    if (vc < 32) m &= (1u << vc) - 1u;  // (vc = 0: no hits)
+    // instruction k of a quad load: the record of lane qa + src(k), chunk chk(k)
+    auto src = [](int k) { return kLoads == 4 ? k : k >> 1; };
+    auto chk = [&](int k) { return kLoads == 4 ? qb : qb + 4 * (k & 1); };
+    auto quad_load = [&](unsigned off, u4(&v)[kLoads]) {
+      const int mine = (int)off;
+#pragma unroll
+      for (int k = 0; k < kLoads; ++k) {
+        const int o = src(k) == 0 ? quad_bcast<0>(mine)
+                    : src(k) == 1 ? quad_bcast<1>(mine)
+                    : src(k) == 2 ? quad_bcast<2>(mine) : quad_bcast<3>(mine);
+        v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, o + 16 * chk(k), 0, 0);
       }
+    };
+    u4 cq[kLoads];
+    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, cq);
+#pragma unroll 1
+    for (int e = 0; e < vmax; ++e) {
+      const bool hit = m & 1u;
+      m >>= 1;
+      q = make_int4(q.y, q.z, q.w, q.x);
+      if (((e + 1) & 3) == 0) {
+        q = qn;
+        qn = vl[(size_t)min((e + 5) >> 2, lastg) * N];
+      }
+      if (((e + 1) & 31) == 0) {
+        m = mn;
+        mn = vm[(size_t)min((e + 33) >> 5, lastw) * N];
+        const int left = vc - (e + 1);
+        if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
+      }
+      u4 nq[kLoads];
+      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, nq);
+#pragma unroll
+      for (int k = 0; k < kLoads; ++k) st[qa + src(k)][chk(k)] = cq[k];
+      __builtin_amdgcn_wave_barrier();
+      u4 mine[kRecVecs<R>];
+#pragma unroll
+      for (int k = 0; k < kRecVecs<R>; ++k) mine[k] = st[lane][k];
+      __builtin_amdgcn_wave_barrier();
+      if (hit) {
+        PairRec<R> cur;
+        __builtin_memcpy(&cur, mine, sizeof(mine));
+        pair_body<R, DIM, WALL, DO_GV, DO_F>(K, cur, B, me, acc);
+      }
+#pragma unroll
+      for (int k = 0; k < kLoads; ++k) cq[k] = nq[k];
     }
-    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
-    return;
+    if (!live) return;
+    if (vok) {
+      pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+      return;
+    }
   }
   const int total = L.count[i];
   const int cnt = min(total, L.kmax);
