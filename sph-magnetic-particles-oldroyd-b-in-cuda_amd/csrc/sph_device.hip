@@ -83,9 +83,11 @@ struct Particles {  // per-particle state that moves with the reorder
   // tests candidates against the cut-off with one 16-byte load each.  Kept
   // current wherever positions change (reorder, VV1, colloid placement, uploads).
   float4 *p4;
-  // (x, y, z, 0) in double, same order: one 32-byte access per gathered
-  // neighbour instead of three scattered 8-byte ones.
+  // FP64 form: (x, y, z, 0) in double, same order: one 32-byte access per
+  // gathered neighbour instead of three scattered 8-byte ones (null in FP32).
   double4 *pd;
+  // FP32 form: fixed-point positions (see Fix), same order (null in FP64).
+  uint4 *pq;
 };
 
 template <class R>
@@ -202,6 +204,64 @@ __device__ __forceinline__ Box box_of(const sph_params_t &P) {
 // rounds d -+ L exactly once, as the subtraction does.
 __device__ __forceinline__ double mimg_r(double d, double L, double iL) {
   return fma(-L, rint(d * iL), d);
+}
+
+// Fixed-point positions of the FP32 form: coordinate a is stored as the 32-bit
+// integer rint((x_a - o_a) 2^32 / span_a).  A periodic axis spans the box
+// (o = 0), so the 32-bit difference of two coordinates is their minimum-image
+// separation; the wall-normal axis spans twice the walled channel, centred on
+// it, so every difference inside the channel is exact.  One 16-byte load
+// carries a particle's position, and a separation costs an integer
+// subtraction and a rounding to float (relative error ~1e-7 - the rounding of
+// the double difference - plus the 2^-32 span quantisation).
+struct Fix {
+  double s[3], o[3];  // 2^32 / span, origin
+  float u[3];         // span / 2^32
+};
+__device__ __forceinline__ Fix fix_of(const sph_params_t &P) {
+  Fix f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double span = a < P.dim ? P.L[a] : 0.0, o = 0.0;
+    if (a == 1 && P.wall) {
+      const double h = P.L[1] + 2.0 * P.wall_width;
+      span = 2.0 * h;
+      o = 0.5 * P.L[1] - h;
+    }
+    f.s[a] = span > 0.0 ? 4294967296.0 / span : 0.0;
+    f.o[a] = o;
+    f.u[a] = (float)(span / 4294967296.0);
+  }
+  return f;
+}
+__device__ __forceinline__ unsigned fix_enc(double x, double s, double o) {
+  return (unsigned)(long long)rint((x - o) * s);
+}
+__device__ __forceinline__ uint4 fix_pos(const Fix &f, double x, double y, double z) {
+  return make_uint4(fix_enc(x, f.s[0], f.o[0]), fix_enc(y, f.s[1], f.o[1]),
+                    fix_enc(z, f.s[2], f.o[2]), 0u);
+}
+template <int DIM>
+__device__ __forceinline__ void fix_sep(const float u[3], uint4 a, uint4 b, float r[3]) {
+  r[0] = (float)(int)(a.x - b.x) * u[0];
+  r[1] = (float)(int)(a.y - b.y) * u[1];
+  r[2] = DIM == 3 ? (float)(int)(a.z - b.z) * u[2] : 0.0f;
+}
+template <int DIM>
+__device__ __forceinline__ float fix_r2(const float u[3], uint4 a, uint4 b) {
+  float r[3];
+  fix_sep<DIM>(u, a, b, r);
+  return DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+}
+
+// Writes the position copies of particle k: float4 (both forms), and double4
+// (FP64 form) or fixed point (FP32 form).
+template <class R>
+__device__ __forceinline__ void store_pos(const Particles<R> &S, const sph_params_t &P, int k,
+                                          double x, double y, double z) {
+  S.p4[k] = make_float4((float)x, (float)y, (float)z, 0.0f);
+  if constexpr (sizeof(R) == 4) S.pq[k] = fix_pos(fix_of(P), x, y, z);
+  else S.pd[k] = make_double4(x, y, z, 0.0);
 }
 
 // Square root and reciprocal of pair separations.  Double: correctly rounded,
@@ -389,11 +449,14 @@ __device__ __forceinline__ void seg_sep(double xi, double yi, double zi, double 
 }
 
 // Cut-off test of a candidate in a strip segment.  FP64 form: (x_i - x_j) + s in
-// double, as the reference.  FP32 form: the same in float on the float copies
-// of the positions (rounding them moves r by ~1e-7 of the box, so only pairs
-// within that distance of the cut-off - where W and gradW vanish to fifth and
-// fourth order - can fall on the other side).  Both forms give a pure function
-// of the positions, identical in every pass that re-scans the strips.
+// double, as the reference.  FP32 form: the fixed-point separation (Fix; the
+// 32-bit wraparound is the segment's periodic shift for every pair within
+// reach, since a box holds at least three cells per axis), rounded to float -
+// the separation every FP32 pass uses, so the strip scans, the Verlet build and
+// the Verlet density walk agree on every hit.  Only pairs within ~1e-7 of the
+// cut-off - where W and gradW vanish to fifth and fourth order - can fall on the
+// other side of the reference's test.  Both forms give a pure function of the
+// positions, identical in every pass that re-scans the strips.
 template <class R, int DIM>
 struct Cand;
 template <int DIM>
@@ -407,15 +470,15 @@ struct Cand<double, DIM> {
 };
 template <int DIM>
 struct Cand<float, DIM> {
-  float4 q;
-  __device__ __forceinline__ void load(const Particles<float> &S, int k) { q = S.p4[k]; }
+  uint4 q;
+  __device__ __forceinline__ void load(const Particles<float> &S, int k) { q = S.pq[k]; }
 };
 template <class R, int DIM>
 struct CandSelf;
 template <int DIM>
 struct CandSelf<double, DIM> {
   double x, y, z, sx, sy, sz;
-  __device__ __forceinline__ void init(const Particles<double> &S, int i) {
+  __device__ __forceinline__ void init(const Particles<double> &S, int i, const sph_params_t &) {
     x = S.x[i];
     y = S.y[i];
     z = DIM == 3 ? S.z[i] : 0.0;
@@ -429,22 +492,18 @@ struct CandSelf<double, DIM> {
 };
 template <int DIM>
 struct CandSelf<float, DIM> {
-  float x, y, z, sx, sy, sz;
-  __device__ __forceinline__ void init(const Particles<float> &S, int i) {
-    const float4 q = S.p4[i];
-    x = q.x;
-    y = q.y;
-    z = q.z;
+  uint4 q;
+  float u[3];
+  __device__ __forceinline__ void init(const Particles<float> &S, int i, const sph_params_t &P) {
+    q = S.pq[i];
+    const Fix f = fix_of(P);
+    u[0] = f.u[0];
+    u[1] = f.u[1];
+    u[2] = f.u[2];
   }
-  __device__ __forceinline__ void shift(double a, double b, double c) {
-    sx = (float)a;
-    sy = (float)b;
-    sz = (float)c;
-  }
+  __device__ __forceinline__ void shift(double, double, double) {}
   __device__ __forceinline__ float r2(const Cand<float, DIM> &c) const {
-    const float rx = (x - c.q.x) + sx, ry = (y - c.q.y) + sy;
-    const float rz = DIM == 3 ? (z - c.q.z) + sz : 0.0f;
-    return DIM == 3 ? rx * rx + ry * ry + rz * rz : rx * rx + ry * ry;
+    return fix_r2<DIM>(u, q, c.q);
   }
 };
 
@@ -786,8 +845,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   dst.x[d] = xk;
   dst.y[d] = yk;
   dst.z[d] = zk;
-  dst.p4[d] = make_float4((float)xk, (float)yk, (float)zk, 0.0f);
-  dst.pd[d] = make_double4(xk, yk, zk, 0.0);
+  store_pos(dst, T.P[sim], d, xk, yk, zk);
 #pragma unroll
   for (int a = 0; a < 3; ++a) dst.v[a][d] = src.v[a][k];
   for (int a = 0; a < nc; ++a) dst.c[a][d] = src.c[a][k];
@@ -985,94 +1043,157 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
   int cnt = 0;
   R d = R(0);
   const bool vok = VL.idx && live && VL.count[i] <= VL.cap;
-  if (VL.idx) {
-    // Verlet list: the hits in list order, W summed as they come; all lanes of
-    // the wave step through the list groups (four entries) together (a lane
-    // past its list or not on the Verlet path has no entries), so index loads
-    // are coalesced.  The hits are marked in the mask words, stored as each
-    // fills.  Positions (double4, 32 bytes) move in two stages: the two lanes
-    // of pair a load, in instruction k, the two 16-byte halves of the position
-    // lane 2a + k needs (one line per lane pair instead of one per lane), then
-    // the halves go through LDS to the lane that needs them.  The loads of
-    // group g + 1 are issued before group g is evaluated; a lane without an
-    // entry asks for an offset past the simulation's positions (zeros, no
-    // memory access).
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    constexpr int kSlot = 3;  // 48-byte slots: conflict-free LDS rows
-    __shared__ u4 stage[kBlock / 64][64][kSlot];
-    u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
-    const int lane = threadIdx.x & 63, hb = lane & 1, pa = lane & ~1;
-    const int vc = vok ? VL.count[i] : 0;
-    int vmax = vc;
-#pragma unroll
-    for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
-    vmax = __builtin_amdgcn_readfirstlane(vmax);
-    const int4 *__restrict__ vl = VL.idx + i;
-    // (uniform by construction; readfirstlane keeps the descriptor scalar)
-    const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
-    const int pe = __builtin_amdgcn_readfirstlane(iend);
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<double4 *>(S.pd + pb), 0, (int)((unsigned)(pe - pb) * 32u), 0x00020000);
-    const unsigned oob = (unsigned)(pe - pb) * 32u;
-    const int lastg = max(vc - 1, 0) >> 2;
-    auto group_load = [&](const int4 &g4, int e0, u4(&v)[4][2]) {
-      const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        // lane pa + k's position offset (broadcast in the pair) plus this lane's half
-        const int mine = (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 32u : oob), ho = 16 * hb;
-        v[u][0] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xA0>(mine) + ho, 0, 0);
-        v[u][1] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xF5>(mine) + ho, 0, 0);
-      }
-    };
-    // q: indices of the current group, qn: of the following group
-    int4 q = vl[0];
-    int4 qn = vl[(size_t)min(1, lastg) * N];
-    u4 cur[4][2];
-    group_load(q, 0, cur);
-    if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
-    unsigned mw = 0u;
-#pragma unroll 1
-    for (int g = 0; g < vmax; g += 4) {
-      u4 nxt[4][2];
-      group_load(qn, g + 4, nxt);
-      q = qn;
-      qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        // half hb of the position of lane pa + k -> slot pa + k
-        st[pa][hb] = cur[u][0];
-        st[pa + 1][hb] = cur[u][1];
-        __builtin_amdgcn_wave_barrier();
-        u4 mine[2];
-        mine[0] = st[lane][0];
-        mine[1] = st[lane][1];
-        __builtin_amdgcn_wave_barrier();
-        if (g + u < vc) {
-          double4 pj;
-          __builtin_memcpy(&pj, mine, sizeof(pj));
-          R r[3];
-          pair_sep<R, DIM, WALL>(bx, xi, yi, zi, pj.x, pj.y, DIM == 3 ? pj.z : 0.0, r);
-          const R r2 =
-              DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-          if (r2 < rcutsq) {
-            d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
-            mw |= 1u << ((g + u) & 31);
-            cnt++;
+  if constexpr (sizeof(R) == 4) {
+    if (VL.idx) {
+      // FP32 form, Verlet list: the hits in list order, W summed as they come.
+      // One 16-byte fixed-point position per entry; the lanes step through the
+      // list groups (four entries) together, so index loads are coalesced, and
+      // the positions of group g + 1 are loaded before group g is evaluated.  A
+      // lane past its list asks for an offset past the simulation's positions
+      // (zeros, no memory access).  The hits are marked in the mask words, stored
+      // as each fills.
+      const int vc = vok ? VL.count[i] : 0;
+      const int4 *__restrict__ vl = VL.idx + i;
+      // (uniform by construction; readfirstlane keeps the descriptor scalar)
+      const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
+      const int pe = __builtin_amdgcn_readfirstlane(iend);
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<uint4 *>(S.pq + pb), 0, (int)((unsigned)(pe - pb) * 16u), 0x00020000);
+      const unsigned oob = (unsigned)(pe - pb) * 16u;
+      const int lastg = max(vc - 1, 0) >> 2;
+      CandSelf<R, DIM> me;
+      me.init(S, i, P);
+      auto group_load = [&](const int4 &g4, int e0, uint4(&v)[4]) {
+        const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
+  #pragma unroll
+        for (int u = 0; u < 4; ++u)
+          v[u] = __builtin_bit_cast(
+              uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                         rs, (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 16u : oob), 0, 0));
+      };
+      // q: indices of the current group, qn: of the following group
+      int4 q = vl[0];
+      int4 qn = vl[(size_t)min(1, lastg) * N];
+      uint4 cur[4];
+      group_load(q, 0, cur);
+      if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+      unsigned mw = 0u;
+  #pragma unroll 1
+      for (int g = 0; g < vc; g += 4) {
+        uint4 nxt[4];
+        group_load(qn, g + 4, nxt);
+        q = qn;
+        qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (g + u < vc) {
+            const R r2 = (R)me.r2(Cand<R, DIM>{cur[u]});
+            if (r2 < rcutsq) {
+              d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+              mw |= 1u << ((g + u) & 31);
+              cnt++;
+            }
           }
         }
+        if (((g + 4) & 31) == 0 || g + 4 >= vc) {
+          VL.mask[(size_t)(g >> 5) * N + i] = mw;
+          mw = 0u;
+        }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
       }
-      if (g < vc && (((g + 4) & 31) == 0 || g + 4 >= vc)) {
-        VL.mask[(size_t)(g >> 5) * N + i] = mw;
-        mw = 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cur[u][0] = nxt[u][0];
-        cur[u][1] = nxt[u][1];
-      }
+      if (!live) return;
     }
-    if (!live) return;
+  } else {
+    if (VL.idx) {
+      // FP64 form, Verlet list: the hits in list order, W summed as they come; all
+      // lanes of the wave step through the list groups (four entries) together (a lane
+      // past its list or not on the Verlet path has no entries), so index loads
+      // are coalesced.  The hits are marked in the mask words, stored as each
+      // fills.  Positions (double4, 32 bytes) move in two stages: the two lanes
+      // of pair a load, in instruction k, the two 16-byte halves of the position
+      // lane 2a + k needs (one line per lane pair instead of one per lane), then
+      // the halves go through LDS to the lane that needs them.  The loads of
+      // group g + 1 are issued before group g is evaluated; a lane without an
+      // entry asks for an offset past the simulation's positions (zeros, no
+      // memory access).
+      typedef unsigned u4 __attribute__((ext_vector_type(4)));
+      constexpr int kSlot = 3;  // 48-byte slots: conflict-free LDS rows
+      __shared__ u4 stage[kBlock / 64][64][kSlot];
+      u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
+      const int lane = threadIdx.x & 63, hb = lane & 1, pa = lane & ~1;
+      const int vc = vok ? VL.count[i] : 0;
+      int vmax = vc;
+  #pragma unroll
+      for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
+      vmax = __builtin_amdgcn_readfirstlane(vmax);
+      const int4 *__restrict__ vl = VL.idx + i;
+      // (uniform by construction; readfirstlane keeps the descriptor scalar)
+      const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
+      const int pe = __builtin_amdgcn_readfirstlane(iend);
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<double4 *>(S.pd + pb), 0, (int)((unsigned)(pe - pb) * 32u), 0x00020000);
+      const unsigned oob = (unsigned)(pe - pb) * 32u;
+      const int lastg = max(vc - 1, 0) >> 2;
+      auto group_load = [&](const int4 &g4, int e0, u4(&v)[4][2]) {
+        const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          // lane pa + k's position offset (broadcast in the pair) plus this lane's half
+          const int mine = (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 32u : oob), ho = 16 * hb;
+          v[u][0] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xA0>(mine) + ho, 0, 0);
+          v[u][1] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xF5>(mine) + ho, 0, 0);
+        }
+      };
+      // q: indices of the current group, qn: of the following group
+      int4 q = vl[0];
+      int4 qn = vl[(size_t)min(1, lastg) * N];
+      u4 cur[4][2];
+      group_load(q, 0, cur);
+      if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+      unsigned mw = 0u;
+  #pragma unroll 1
+      for (int g = 0; g < vmax; g += 4) {
+        u4 nxt[4][2];
+        group_load(qn, g + 4, nxt);
+        q = qn;
+        qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          // half hb of the position of lane pa + k -> slot pa + k
+          st[pa][hb] = cur[u][0];
+          st[pa + 1][hb] = cur[u][1];
+          __builtin_amdgcn_wave_barrier();
+          u4 mine[2];
+          mine[0] = st[lane][0];
+          mine[1] = st[lane][1];
+          __builtin_amdgcn_wave_barrier();
+          if (g + u < vc) {
+            double4 pj;
+            __builtin_memcpy(&pj, mine, sizeof(pj));
+            R r[3];
+            pair_sep<R, DIM, WALL>(bx, xi, yi, zi, pj.x, pj.y, DIM == 3 ? pj.z : 0.0, r);
+            const R r2 =
+                DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+            if (r2 < rcutsq) {
+              d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+              mw |= 1u << ((g + u) & 31);
+              cnt++;
+            }
+          }
+        }
+        if (g < vc && (((g + 4) & 31) == 0 || g + 4 >= vc)) {
+          VL.mask[(size_t)(g >> 5) * N + i] = mw;
+          mw = 0u;
+        }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cur[u][0] = nxt[u][0];
+          cur[u][1] = nxt[u][1];
+        }
+      }
+      if (!live) return;
+    }
   }
   if (!vok) {
     R d_over = R(0);  // W of the hits past the list capacity, in scan order
@@ -1080,7 +1201,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     // Phase 1: the candidate test only.  Four candidates' positions are loaded
     // before any is tested, so their latencies overlap.
     CandSelf<R, DIM> me;
-    me.init(S, i);
+    me.init(S, i, P);
     auto test = [&](int k, const Cand<R, DIM> &c) {
       const R r2 = (R)me.r2(c);
       if (r2 < rcutsq && k != i) {
@@ -1124,23 +1245,34 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       for (int g = 0; g < stored; g += 4) {
         int4 q = out[(size_t)(g >> 2) * N];
         const int m = min(4, stored - g);
-        double xj[4], yj[4], zj[4];
         const int jj[4] = {q.x, q.y, q.z, q.w};
+        if constexpr (sizeof(R) == 4) {
+          uint4 pj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int j = u < m ? jj[u] : i;
-          xj[u] = X[j];
-          yj[u] = Y[j];
-          zj[u] = DIM == 3 ? Z[j] : 0.0;
-        }
+          for (int u = 0; u < 4; ++u) pj[u] = S.pq[u < m ? jj[u] : i];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (u >= m) break;
-          R r[3];
-          pair_sep<R, DIM, WALL>(bx, xi, yi, zi, xj[u], yj[u], zj[u], r);
-          const R r2 =
-              DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-          d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+          for (int u = 0; u < 4; ++u) {
+            if (u >= m) break;
+            d = d + sph_W<R>(pair_sqrt((R)me.r2(Cand<R, DIM>{pj[u]})), rinv, cw);
+          }
+        } else {
+          double xj[4], yj[4], zj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = u < m ? jj[u] : i;
+            xj[u] = X[j];
+            yj[u] = Y[j];
+            zj[u] = DIM == 3 ? Z[j] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (u >= m) break;
+            R r[3];
+            pair_sep<R, DIM, WALL>(bx, xi, yi, zi, xj[u], yj[u], zj[u], r);
+            const R r2 =
+                DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+            d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+          }
         }
       }
       d = d + d_over;
@@ -1182,7 +1314,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   // The cut-off test of the density pass (float positions in the FP32 form),
   // four candidates' loads in flight at a time.
   CandSelf<R, DIM> me;
-  me.init(S, i);
+  me.init(S, i, P);
   auto test = [&](int k, const Cand<R, DIM> &c) {
     if ((R)me.r2(c) < rv2 && k != i) {
       buf = make_int4(buf.y, buf.z, buf.w, k);
@@ -1233,13 +1365,11 @@ __global__ void k_fill(int *a, int n, int v) {
 }
 
 // Float copies of the positions (after an upload of x, y or z).
-__global__ __launch_bounds__(kBlock) void k_pack_pos(const double *x, const double *y,
-                                                     const double *z, float4 *p4, double4 *pd,
-                                                     int n) {
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pack_pos(SimTable T, Particles<R> S) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  p4[i] = make_float4((float)x[i], (float)y[i], (float)z[i], 0.0f);
-  pd[i] = make_double4(x[i], y[i], z[i], 0.0);
+  if (i >= T.ntotal) return;
+  store_pos(S, T.P[sim_of(T, i)], i, S.x[i], S.y[i], S.z[i]);
 }
 
 template <class R>
@@ -1645,7 +1775,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
     int cx, cy, cz;
     cell_coords<DIM, WALL>(P, me.x, me.y, me.z, cx, cy, cz);
     CandSelf<R, DIM> cme;
-    cme.init(S, i);
+    cme.init(S, i, P);
     for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
                                                       double sz) {
       cme.shift(sx, sy, sz);
@@ -1832,8 +1962,7 @@ __global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Deri
   S.x[i] = xi;
   S.y[i] = yi;
   if (DIM == 3) S.z[i] = zi;
-  S.p4[i] = make_float4((float)xi, (float)yi, (float)zi, 0.0f);
-  S.pd[i] = make_double4(xi, yi, zi, 0.0);
+  store_pos(S, P, i, xi, yi, zi);
   displacement_check(V, P, sim, i, xi, yi, zi);
 }
 
@@ -1963,8 +2092,7 @@ __global__ __launch_bounds__(kBlock) void k_coll_place(SimTable T, Particles<R> 
   S.x[p] = X;
   S.y[p] = Y;
   if (P.dim == 3) S.z[p] = Z;
-  S.p4[p] = make_float4((float)X, (float)Y, (float)Z, 0.0f);
-  S.pd[p] = make_double4(X, Y, Z, 0.0);
+  store_pos(S, P, p, X, Y, Z);
   displacement_check(V, P, sim, p, X, Y, Z);
 }
 
@@ -2186,7 +2314,9 @@ class DeviceImpl final : public Device {
       for (int a = 0; a < 6; ++a) HIPCHK(c_[b][a].alloc(N_));
       HIPCHK(mass_[b].alloc(N_)); HIPCHK(type_[b].alloc(N_)); HIPCHK(id_[b].alloc(N_));
     }
-    HIPCHK(p4_.alloc(N_)); HIPCHK(pd_.alloc(N_));
+    HIPCHK(p4_.alloc(N_));
+    if (sizeof(R) == 4) HIPCHK(pq_.alloc(N_));
+    else HIPCHK(pd_.alloc(N_));
     HIPCHK(newpos_.alloc(N_)); HIPCHK(ptmp_.alloc(N_));
     HIPCHK(dens_.alloc(N_)); HIPCHK(press_.alloc(N_));
     for (int a = 0; a < 3; ++a) HIPCHK(f_[a].alloc(N_));
@@ -2286,15 +2416,6 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemcpy(x_[0].p, xyz[0].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(y_[0].p, xyz[1].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(z_[0].p, xyz[2].data(), N_ * sizeof(double), hipMemcpyHostToDevice));
-    {
-      std::vector<float4> q(N_);
-      for (int i = 0; i < N_; ++i)
-        q[i] = make_float4((float)xyz[0][i], (float)xyz[1][i], (float)xyz[2][i], 0.0f);
-      HIPCHK(hipMemcpy(p4_.p, q.data(), N_ * sizeof(float4), hipMemcpyHostToDevice));
-      std::vector<double4> qd(N_);
-      for (int i = 0; i < N_; ++i) qd[i] = make_double4(xyz[0][i], xyz[1][i], xyz[2][i], 0.0);
-      HIPCHK(hipMemcpy(pd_.p, qd.data(), N_ * sizeof(double4), hipMemcpyHostToDevice));
-    }
     auto put_all = [&](R *dst, int field) -> int {
       for (int s = 0; s < S; ++s) {
         const HostState &h = *hs[s];
@@ -2408,6 +2529,8 @@ class DeviceImpl final : public Device {
     if ((rc = build_blocks(err))) return rc;
     cur_ = 0;
     have_cells_ = false;
+    k_pack_pos<R><<<blocks(N_), kBlock, 0, stream_>>>(table(), parts(0));  // position copies
+    HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     return SPH_OK;
   }
@@ -2851,9 +2974,7 @@ class DeviceImpl final : public Device {
         }
         double *dst = k == F_X ? x_[cur_].p : (k == F_Y ? y_[cur_].p : z_[cur_].p);
         HIPCHK(hipMemcpy(dst + b, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
-        k_pack_pos<<<blocks(N_), kBlock, 0, stream_>>>(x_[cur_].p, y_[cur_].p, z_[cur_].p, p4_.p,
-                                                       pd_.p,
-                                                       N_);
+        k_pack_pos<R><<<blocks(N_), kBlock, 0, stream_>>>(table(), parts(cur_));
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
         list_valid_ = false;
@@ -3028,6 +3149,7 @@ class DeviceImpl final : public Device {
     S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
     S.p4 = p4_.p;
     S.pd = pd_.p;
+    S.pq = pq_.p;
     return S;
   }
   NList nlist() { return NList{nl_idx_.p, nl_count_.p, nl_queue_.p, nl_qlen_.p, kmax_}; }
@@ -3129,6 +3251,7 @@ class DeviceImpl final : public Device {
   DevBuf<int> type_[2], id_[2];
   DevBuf<float4> p4_;
   DevBuf<double4> pd_;
+  DevBuf<uint4> pq_;
   DevBuf<int> newpos_;  // old sorted position -> new, of the last stand-alone reorder
   DevBuf<R> ptmp_;
   DevBuf<R> dens_, press_, dinv_, f_[3], dcdt_[6], gv_[9];
