@@ -1674,10 +1674,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
         v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, o + 16 * chk(k), 0, 0);
       }
     };
-    u4 cq[kLoads];
-    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, cq);
-#pragma unroll 1
-    for (int e = 0; e < vmax; ++e) {
+    // entry e: evaluate the chunks in cur, quad-load entry e + 1 into nxt
+    auto entry = [&](int e, const u4(&cur)[kLoads], u4(&nxt)[kLoads]) {
       const bool hit = m & 1u;
       m >>= 1;
       q = make_int4(q.y, q.z, q.w, q.x);
@@ -1691,22 +1689,27 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
         const int left = vc - (e + 1);
         if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
       }
-      u4 nq[kLoads];
-      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, nq);
+      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, nxt);
 #pragma unroll
-      for (int k = 0; k < kLoads; ++k) st[qa + src(k)][chk(k)] = cq[k];
+      for (int k = 0; k < kLoads; ++k) st[qa + src(k)][chk(k)] = cur[k];
       __builtin_amdgcn_wave_barrier();
       u4 mine[kRecVecs<R>];
 #pragma unroll
       for (int k = 0; k < kRecVecs<R>; ++k) mine[k] = st[lane][k];
       __builtin_amdgcn_wave_barrier();
       if (hit) {
-        PairRec<R> cur;
-        __builtin_memcpy(&cur, mine, sizeof(mine));
-        pair_body<R, DIM, WALL, DO_GV, DO_F>(K, cur, B, me, acc);
+        PairRec<R> rj;
+        __builtin_memcpy(&rj, mine, sizeof(mine));
+        pair_body<R, DIM, WALL, DO_GV, DO_F>(K, rj, B, me, acc);
       }
-#pragma unroll
-      for (int k = 0; k < kLoads; ++k) cq[k] = nq[k];
+    };
+    // two entries per trip with the chunk buffers swapping roles (no copies)
+    u4 ca[kLoads], cb[kLoads];
+    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, ca);
+#pragma unroll 1
+    for (int e = 0; e < vmax; e += 2) {
+      entry(e, ca, cb);
+      if (e + 1 < vmax) entry(e + 1, cb, ca);
     }
     if (!live) return;
     if (vok) {
