@@ -140,7 +140,7 @@ __device__ __forceinline__ int sim_of(const SimTable &T, int i) {
 }
 
 // Verlet lists: every particle within rcut + skin at the last rebuild, in the
-// order of a 5 x 5 (x 5) cell scan.  Until any particle of a simulation has
+// order of a scan over the cells it can reach.  Until any particle of a simulation has
 // moved more than skin / 2 from its position at the rebuild (ref), the list
 // still holds every pair closer than rcut, so the per-step passes test only the
 // list instead of 27 cells.  rebuild[s] asks for a rebuild of simulation s in
@@ -392,18 +392,34 @@ __device__ __forceinline__ void for_each_strip(const sph_params_t &P, const int 
   }
 }
 
-// The 5 x 5 (x 5) cell neighbourhood of the Verlet build: calls fn(k0, k1, sx,
-// sy, sz) for each contiguous sorted range of rows y-2..y+2 (z-2..z+2) and
-// cells x-2..x+2, split at the periodic seam as for_each_strip does.  Needs at
-// least 5 cells along every periodic axis (checked on the host), so no cell is
-// visited twice.
+// The cells a Verlet list can reach: calls fn(k0, k1, sx, sy, sz) for each
+// contiguous sorted range of cells cx - lo .. cx + hi per axis, rows in
+// (z, y) order, split at the periodic seam as for_each_strip does.  lo and hi
+// are 1, or 2 where the particle lies closer than rv - (cell size) to that
+// face of its cell (cells are at least rcut wide and rv = rcut + skin <= 2
+// rcut), so a build scans ~3 x 3 x 3 cells instead of 5 x 5 x 5; the hits
+// (within rcut, so within the 3 x 3 x 3 block) come in the order of the strip
+// scan.  Needs at least 5 cells along every periodic axis (checked on the
+// host), so no cell is visited twice.
 template <int DIM, bool WALL, class Fn>
-__device__ __forceinline__ void for_each_strip5(const sph_params_t &P, const int *__restrict__ cs,
-                                                int cx, int cy, int cz, Fn &&fn) {
+__device__ __forceinline__ void for_each_reach(const sph_params_t &P, const int *__restrict__ cs,
+                                               double x, double y, double z, int cx, int cy,
+                                               int cz, double rv, Fn &&fn) {
   const int Nx = P.Ncells[0], Ny = P.Ncells[1], Nz = P.Ncells[2];
   if (cx < 0 || cx > Nx || cy < -1 || cy > Ny || (DIM == 3 && (cz < 0 || cz > Nz))) return;
+  // reach below / above the particle's cell along one axis (a small margin
+  // keeps the scan a superset under rounding)
+  const double rvm = rv * (1.0 + 1e-6);
+  auto reach = [&](double u, double h, int c, int &lo, int &hi) {
+    lo = max(1, min(2, c - (int)floor((u - rvm) / h)));
+    hi = max(1, min(2, (int)floor((u + rvm) / h) - c));
+  };
+  int xlo, xhi, ylo, yhi, zlo = 0, zhi = 0;
+  reach(x, P.cell_size[0], cx, xlo, xhi);
+  reach(WALL ? y + P.wall_width : y, P.cell_size[1], cy, ylo, yhi);
+  if (DIM == 3) reach(z, P.cell_size[2], cz, zlo, zhi);
   const double Lx = P.L[0];
-  int a0 = cx - 2, a1 = cx + 3, b0 = 0, b1 = 0;
+  int a0 = cx - xlo, a1 = cx + xhi + 1, b0 = 0, b1 = 0;
   double sa = 0.0, sb = 0.0;
   if (a0 < 0) {          // [Nx + a0, Nx) shifted +L, then [0, a1)
     b1 = a1;
@@ -415,14 +431,14 @@ __device__ __forceinline__ void for_each_strip5(const sph_params_t &P, const int
     a1 = Nx;
     sb = -Lx;
   }
-  constexpr int kRows = DIM == 3 ? 25 : 5;
+  const int ny = ylo + yhi + 1, nrows = ny * (zlo + zhi + 1);
 #pragma unroll 1
-  for (int s = 0; s < 2 * kRows; ++s) {
+  for (int s = 0; s < 2 * nrows; ++s) {
     const int part = s & 1;
     if (part && b1 == 0) continue;
     const int row = s >> 1;
-    int y = cy + row % 5 - 2;
-    int z = DIM == 3 ? cz + row / 5 - 2 : 0;
+    int y = cy + row % ny - ylo;
+    int z = DIM == 3 ? cz + row / ny - zlo : 0;
     double sy = 0.0, sz = 0.0;
     if (DIM == 3) {
       if (z < 0) { z += Nz; sz = P.L[2]; }
@@ -1286,9 +1302,9 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 }
 
 // Verlet list build of the simulations being rebuilt: every other particle
-// within rcut + skin over the 5 x 5 (x 5) cell neighbourhood (cells are at
-// least rcut wide and skin <= rcut), in scan order; the positions at the build
-// become the reference of the displacement checks.  Lists that overflow their
+// within rcut + skin over the cells it can reach (for_each_reach), in scan
+// order; the positions at the build become the reference of the displacement
+// checks.  Lists that overflow their
 // capacity raise overflow[sim]: those particles use the strip scan, and the
 // simulation is rebuilt again next step.
 template <int DIM, bool WALL, class R>
@@ -1322,8 +1338,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
       if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
     }
   };
-  for_each_strip5<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
-                                                     double sz) {
+  for_each_reach<DIM, WALL>(P, cs, xi, yi, zi, cx, cy, cz, rv, [&](int k0, int k1, double sx,
+                                                                    double sy, double sz) {
     me.shift(sx, sy, sz);
     int k = k0;
 #pragma unroll 1
