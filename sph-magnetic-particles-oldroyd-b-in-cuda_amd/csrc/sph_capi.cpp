@@ -533,13 +533,16 @@ int sph_batch_observables(sph_batch_handle b, double *out) {
   BATCH_READY(b);
   if (!out) return SPH_ERR_ARG;
   const int S = (int)b->sims.size();
-  std::vector<double> ke(S), sums(6 * S);
+  std::vector<double> ke(S), sums(6 * S), fw(6 * S);
   int rc;
   if ((rc = sph_batch_kinetic_energy(b, ke.data()))) return rc;
   if ((rc = b->dev->observable_sums(sums.data(), b->err))) return rc;
+  if ((rc = b->dev->wall_forces(fw.data(), b->err))) return rc;
   for (int s = 0; s < S; ++s) {
     sph::HostState &hs = b->sims[s];
-    if (hs.p.wall && (rc = b->dev->download_field(s, hs, "fx_wall", b->err))) return rc;
+    if (hs.p.wall)
+      for (int a = 0; a < 3; ++a)
+        for (int w = 0; w < 2; ++w) hs.fwall[a][w] = fw[6 * s + 2 * a + w];
     finish_observables(hs.p, b->step, ke[s], &sums[6 * s], hs.fwall[0], out + 8 * s);
   }
   return SPH_OK;

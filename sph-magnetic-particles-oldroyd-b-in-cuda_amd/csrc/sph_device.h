@@ -51,6 +51,9 @@ class Device {
   // [0] sum(cxx-cyy) [1] sum(cxy) over fluid, [2] fluid count, [3] sum sin 2phi,
   // [4] sum cos 2phi, [5] count of colloid pairs within the magnetic cut-off.
   virtual int observable_sums(double *out, std::string &err) = 0;
+  // Wall force sums of every simulation: out[6 s + 2 a + w] is axis a of wall w
+  // (0 bottom, 1 top), one transfer for the whole batch.
+  virtual int wall_forces(double *out, std::string &err) = 0;
   virtual int synchronize(std::string &err) = 0;
   // Checks the device error flags (a particle outside the grid); resets them.
   virtual int check_flags(std::string &err) = 0;

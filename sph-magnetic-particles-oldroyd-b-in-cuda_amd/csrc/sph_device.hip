@@ -2898,6 +2898,13 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
+  int wall_forces(double *out, std::string &err) override {
+    HIPCHK(hipMemcpyAsync(out, fwall_.p, 6 * (size_t)nsim() * sizeof(double),
+                          hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return SPH_OK;
+  }
+
   int kinetic_energy(double *ke, std::string &err) override {
     const int S = nsim();
     k_ke_partial<R><<<dim3(max_parts_, S), kBlock, 0, stream_>>>(table(), parts(cur_),
