@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       me.init(S, i, P);
       auto group_load = [&](const int4 &g4, int e0, uint4(&v)[4]) {
         const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u)
           v[u] = __builtin_bit_cast(
               uint4, __builtin_amdgcn_raw_buffer_load_b128(
@@ -1115,13 +1115,13 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       group_load(q, 0, cur);
       if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
       unsigned mw = 0u;
-  #pragma unroll 1
+#pragma unroll 1
       for (int g = 0; g < vc; g += 4) {
         uint4 nxt[4];
         group_load(qn, g + 4, nxt);
         q = qn;
         qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (g + u < vc) {
             const R r2 = (R)me.r2(Cand<R, DIM>{cur[u]});
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
           VL.mask[(size_t)(g >> 5) * N + i] = mw;
           mw = 0u;
         }
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
       }
       if (!live) return;
@@ -1161,7 +1161,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       const int lane = threadIdx.x & 63, hb = lane & 1, pa = lane & ~1;
       const int vc = vok ? VL.count[i] : 0;
       int vmax = vc;
-  #pragma unroll
+#pragma unroll
       for (int o = 32; o; o >>= 1) vmax = max(vmax, __shfl_xor(vmax, o));
       vmax = __builtin_amdgcn_readfirstlane(vmax);
       const int4 *__restrict__ vl = VL.idx + i;
@@ -1174,7 +1174,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       const int lastg = max(vc - 1, 0) >> 2;
       auto group_load = [&](const int4 &g4, int e0, u4(&v)[4][2]) {
         const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
           // lane pa + k's position offset (broadcast in the pair) plus this lane's half
           const int mine = (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 32u : oob), ho = 16 * hb;
@@ -1189,13 +1189,13 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       group_load(q, 0, cur);
       if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
       unsigned mw = 0u;
-  #pragma unroll 1
+#pragma unroll 1
       for (int g = 0; g < vmax; g += 4) {
         u4 nxt[4][2];
         group_load(qn, g + 4, nxt);
         q = qn;
         qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
           // half hb of the position of lane pa + k -> slot pa + k
           st[pa][hb] = cur[u][0];
@@ -1223,7 +1223,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
           VL.mask[(size_t)(g >> 5) * N + i] = mw;
           mw = 0u;
         }
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
           cur[u][0] = nxt[u][0];
           cur[u][1] = nxt[u][1];
