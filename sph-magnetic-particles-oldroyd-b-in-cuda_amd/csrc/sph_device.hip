@@ -216,6 +216,7 @@ __device__ __forceinline__ double mimg_r(double d, double L, double iL) {
 // the double difference - plus the 2^-32 span quantisation).
 struct Fix {
   double s[3], o[3];  // 2^32 / span, origin
+  double ud[3];       // span / 2^32
   float u[3];         // span / 2^32
 };
 __device__ __forceinline__ Fix fix_of(const sph_params_t &P) {
@@ -230,7 +231,8 @@ __device__ __forceinline__ Fix fix_of(const sph_params_t &P) {
     }
     f.s[a] = span > 0.0 ? 4294967296.0 / span : 0.0;
     f.o[a] = o;
-    f.u[a] = (float)(span / 4294967296.0);
+    f.ud[a] = span / 4294967296.0;
+    f.u[a] = (float)f.ud[a];
   }
   return f;
 }
@@ -557,15 +559,89 @@ __device__ __forceinline__ void pair_sep(const Box &X, double xi, double yi, dou
 // The parameters the pair body reads, copied out of the parameter block once
 // per thread; read through the parameter pointer inside the neighbour loop
 // they were re-loaded for every neighbour.
-template <class R>
+// Pair records with fixed-point positions: the FP32 form in 2D (whose record
+// then fits 32 bytes).  The others carry double positions.
+template <class R, int DIM>
+constexpr bool kFixRec = sizeof(R) == 4 && DIM == 2;
+
+// (With fixed-point records only the fixed-point scale stays live; the box and
+// the fixed-point origins are re-derived from P in the rare wall / colloid
+// branches, which keeps the loop's scalar registers for the hot values.)
+template <class R, int DIM>
 struct PairK {
-  Box X;
+  const sph_params_t *P;
+  Box X;       // double positions: the minimum image of every pair
+  float u[3];  // fixed-point positions: span / 2^32
   double y_bottom, y_top;
   R rcut_inv, c_gradw, bmax, nkT, a, b, V_bottom, V_top, coll_R;
-  __device__ explicit PairK(const sph_params_t &P)
-      : X(box_of(P)), y_bottom(P.y_bottom), y_top(P.y_top), rcut_inv((R)P.rcut_inv),
-        c_gradw((R)P.c_gradw), bmax((R)P.beta_max), nkT((R)P.nkT), a((R)P.a), b((R)P.b),
-        V_bottom((R)P.V_bottom), V_top((R)P.V_top), coll_R((R)P.coll_R) {}
+  __device__ explicit PairK(const sph_params_t &P_)
+      : P(&P_), X(kFixRec<R, DIM> ? Box{} : box_of(P_)), y_bottom(P_.y_bottom),
+        y_top(P_.y_top), rcut_inv((R)P_.rcut_inv), c_gradw((R)P_.c_gradw),
+        bmax((R)P_.beta_max), nkT((R)P_.nkT), a((R)P_.a), b((R)P_.b),
+        V_bottom((R)P_.V_bottom), V_top((R)P_.V_top), coll_R((R)P_.coll_R) {
+    if constexpr (kFixRec<R, DIM>) {
+      const Fix f = fix_of(P_);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[k] = f.u[k];
+    }
+  }
+  __device__ __forceinline__ Box box() const { return kFixRec<R, DIM> ? box_of(*P) : X; }
+};
+
+// What the pair pass needs of a neighbour j, in registers: position (the
+// 32-bit fixed-point coordinates or doubles, as the record holds it), velocity,
+// w = 1/d for fluid or -type for wall/colloid particles, and q = Pi/d^2
+// (Pi = p I - nkT c; xx xy yy xz yz zz) for fluid or the raw conformation
+// tensor c for wall/colloid particles (the only parts the reference's force and
+// gradient kernels read of such a partner, src/kernel_forces.cu:196-233,
+// 320-336).  Components a dimension lacks are zero.
+template <class R>
+struct PairVal;
+template <>
+struct PairVal<float> {
+  unsigned X[3];  // fixed-point records
+  double x[3];    // double-position records
+  float w, v[3], q[6];
+};
+template <>
+struct PairVal<double> {
+  double x[3];
+  double w, v[3], q[6];
+};
+
+// Coordinate a of neighbour j's position in double: the stored double, or the
+// fixed-point coordinate mapped back (within 2^-33 of the span of the double
+// position).
+template <class R, int DIM>
+__device__ __forceinline__ double pos_of(const PairK<R, DIM> &K, const PairVal<R> &rj, int a) {
+  if constexpr (kFixRec<R, DIM>) {
+    const Fix f = fix_of(*K.P);
+    return fma((double)rj.X[a], f.ud[a], f.o[a]);
+  } else {
+    return rj.x[a];
+  }
+}
+
+// Everything about particle i that the pair body reads.  Its position: the
+// fixed-point coordinates or the doubles, as the records hold them.
+template <class R>
+struct PairSelf {
+  double x, y, z;  // double-position records
+  unsigned X[3];   // fixed-point records
+  int t;
+  R v[3];
+  R dinv, p;
+  R pd[6];
+  // coordinate a as a double (fixed-point values mapped back)
+  template <int DIM>
+  __device__ __forceinline__ double pos(const PairK<R, DIM> &K, int a) const {
+    if constexpr (kFixRec<R, DIM>) {
+      const Fix f = fix_of(*K.P);
+      return fma((double)X[a], f.ud[a], f.o[a]);
+    } else {
+      return a == 0 ? x : a == 1 ? y : z;
+    }
+  }
 };
 
 // Relative velocity v_ij with Morris boundary conditions for wall and colloid
@@ -573,10 +649,10 @@ struct PairK {
 // src/kernel_forces.cu:196-419, 533-822).  Called only when ti == tj == 0 or
 // ti != tj.
 template <class R, int DIM, bool WALL>
-__device__ __forceinline__ void morris_vij(const PairK<R> &K, const Bodies &B, int ti, int tj,
-                                           double xi, double yi, double zi, double xj, double yj,
-                                           double zj, const R vi[3], const R vj[3],
-                                           const R rij[3], R v[3]) {
+__device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const Bodies &B, int ti,
+                                           int tj, double xi, double yi, double zi,
+                                           double xj, double yj, double zj, const R vi[3],
+                                           const R vj[3], const R rij[3], R v[3]) {
   const R bmax = K.bmax;
   if (ti == tj) {
 #pragma unroll
@@ -601,7 +677,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R> &K, const Bodies &B, i
     } else {
       const int c = tj - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.X, xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
+      pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
                              DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
@@ -641,7 +717,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R> &K, const Bodies &B, i
     } else {
       const int c = ti - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.X, xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
+      pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
                              DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
@@ -920,48 +996,71 @@ struct NList {
   int kmax;    // multiple of 4
 };
 
-// What the pair pass gathers about a neighbour j, packed into one record so a
-// neighbour costs one 64-byte line (FP32 form; two for FP64) instead of one
-// line per structure-of-arrays field: position, velocity, w = 1/d for fluid or
-// -type for wall/colloid particles, and q = Pi/d^2 (Pi = p I - nkT c) for fluid
-// or the raw conformation tensor c for wall/colloid particles (the only parts
-// the reference's force and gradient kernels read of such a partner,
-// src/kernel_forces.cu:196-233, 320-336).
-template <class R>
+// The stored record: the fields the dimension uses, packed so that a neighbour
+// costs as few 16-byte chunks as possible (gathered as whole chunks):
+//   FP32 3D  [x y] [z vx vy] [vz w qxx qxy] [qyy qxz qyz qzz]        64 B
+//   FP32 2D  [X Y w vx] [vy qxx qxy qyy]                              32 B
+//   FP64 3D  [x y] [z vx] [vy vz] [w qxx] [qxy qyy] [qxz qyz] [qzz]  128 B
+//   FP64 2D  [x y] [vx vy] [w qxx] [qxy qyy]                          64 B
+template <class R, int DIM>
 struct PairRec;
 template <>
-struct alignas(64) PairRec<float> {
-  double x, y, z;
-  float v[3];
-  float w;
-  float q[6];
+struct alignas(64) PairRec<float, 3> {
+  double x[3];
+  float v[3], w, q[6];
 };
 template <>
-struct alignas(128) PairRec<double> {
-  double x, y, z;
-  double v[3];
-  double w;
-  double q[6];
-  double pad[3];
+struct alignas(32) PairRec<float, 2> {
+  unsigned X[2];
+  float w, v[2], q[3];
 };
-static_assert(sizeof(PairRec<float>) == 64, "FP32 pair record must be one 64-byte line");
-static_assert(sizeof(PairRec<double>) == 128, "FP64 pair record must be two lines");
+template <>
+struct alignas(128) PairRec<double, 3> {
+  double x[3], v[3], w, q[6];
+};
+template <>
+struct alignas(64) PairRec<double, 2> {
+  double x[2], v[2], w, q[3];
+};
+static_assert(sizeof(PairRec<float, 3>) == 64 && sizeof(PairRec<float, 2>) == 32 &&
+                  sizeof(PairRec<double, 3>) == 128 && sizeof(PairRec<double, 2>) == 64,
+              "pair records are whole 16-byte chunks");
 
-// 16-byte vectors holding the used part of a record (the FP64 record's padding
-// is skipped).
-template <class R>
-constexpr int kRecVecs = (int)((offsetof(PairRec<R>, q) + 6 * sizeof(R) + 15) / 16);
+// 16-byte chunks holding the used part of a record (alignment padding skipped).
+template <class R, int DIM>
+constexpr int kRecVecs = DIM == 3 ? (sizeof(R) == 4 ? 4 : 7) : (sizeof(R) == 4 ? 2 : 4);
+
+template <class R, int DIM>
+__device__ __forceinline__ PairVal<R> unpack(const PairRec<R, DIM> &r) {
+  PairVal<R> o;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if constexpr (kFixRec<R, DIM>) o.X[a] = a < DIM ? r.X[a] : 0u;
+    else o.x[a] = a < DIM ? r.x[a] : 0.0;
+    o.v[a] = a < DIM ? r.v[a] : R(0);
+  }
+  o.w = r.w;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) o.q[k] = (DIM == 3 || k < 3) ? r.q[k] : R(0);
+  return o;
+}
+
+// Record from its chunks (as loaded).
+template <class R, int DIM, class V>
+__device__ __forceinline__ PairVal<R> unpack_chunks(const V (&c)[kRecVecs<R, DIM>]) {
+  PairRec<R, DIM> r;
+  __builtin_memcpy(&r, c, sizeof(c));
+  return unpack<R, DIM>(r);
+}
 
 // The record of particle j (global index).
-template <class R>
-__device__ __forceinline__ PairRec<R> load_rec(const PairRec<R> *__restrict__ rec, int j) {
+template <class R, int DIM>
+__device__ __forceinline__ PairVal<R> load_rec(const PairRec<R, DIM> *__restrict__ rec, int j) {
   const float4 *__restrict__ p = reinterpret_cast<const float4 *>(rec + j);
-  float4 v[kRecVecs<R>];
+  float4 v[kRecVecs<R, DIM>];
 #pragma unroll
-  for (int k = 0; k < kRecVecs<R>; ++k) v[k] = p[k];
-  PairRec<R> r;
-  __builtin_memcpy(&r, v, sizeof(v));
-  return r;
+  for (int k = 0; k < kRecVecs<R, DIM>; ++k) v[k] = p[k];
+  return unpack_chunks<R, DIM>(v);
 }
 
 // Value of lane 4 (l / 4) + K of the quad of lane l (DPP quad_perm [K,K,K,K]).
@@ -977,46 +1076,44 @@ __device__ __forceinline__ int quad_pick(int x) {
 
 // The records of one simulation as a buffer resource (a scalar descriptor:
 // base, size in bytes, raw addressing; 0x00020000 is the gfx9 word 3).  Record
-// loads through it are whole dwordx4 buffer loads - a record costs four L1
-// accesses per lane (FP32; seven FP64) where field-wise global loads cost five -
-// and an offset past the simulation's records reads zeros instead of faulting.
-// Offsets are 32-bit: the host limits a simulation to 2^32 - 1 bytes of records.
+// loads through it are whole dwordx4 buffer loads, and an offset past the
+// simulation's records reads zeros instead of faulting.  Offsets are 32-bit:
+// the host limits a simulation to 2^32 - 1 bytes of records.
 struct RecBuf {
   __amdgpu_buffer_rsrc_t rs;
 };
-template <class R>
-__device__ __forceinline__ RecBuf rec_buf(const PairRec<R> *rec, const SimTable &T, int sim) {
+template <class R, int DIM>
+__device__ __forceinline__ RecBuf rec_buf(const PairRec<R, DIM> *rec, const SimTable &T,
+                                          int sim) {
   // uniform by construction (one simulation per workgroup); readfirstlane keeps
   // the descriptor in scalar registers where the compiler cannot prove it
   const int b = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
   const int n = __builtin_amdgcn_readfirstlane(T.pbeg[sim + 1]) - b;
   RecBuf r;
-  r.rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<PairRec<R> *>(rec + b), 0,
-                                           (int)((unsigned)n * (unsigned)sizeof(PairRec<R>)),
-                                           0x00020000);
+  r.rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<PairRec<R, DIM> *>(rec + b), 0,
+      (int)((unsigned)n * (unsigned)sizeof(PairRec<R, DIM>)), 0x00020000);
   return r;
 }
 
 // The record of particle jl (index within the simulation) through the buffer.
-template <class R>
-__device__ __forceinline__ PairRec<R> load_rec(const RecBuf &rb, int jl) {
+template <class R, int DIM>
+__device__ __forceinline__ PairVal<R> load_rec(const RecBuf &rb, int jl) {
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
-  const unsigned off = (unsigned)jl * (unsigned)sizeof(PairRec<R>);
-  u4 v[kRecVecs<R>];
+  const unsigned off = (unsigned)jl * (unsigned)sizeof(PairRec<R, DIM>);
+  u4 v[kRecVecs<R, DIM>];
 #pragma unroll
-  for (int k = 0; k < kRecVecs<R>; ++k)
+  for (int k = 0; k < kRecVecs<R, DIM>; ++k)
     v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, (int)(off + 16u * k), 0, 0);
-  PairRec<R> r;
-  __builtin_memcpy(&r, v, sizeof(v));
-  return r;
+  return unpack_chunks<R, DIM>(v);
 }
 
 // Pressure p = c^2 (m d - rho0) (src/kernel_pressures.cu:21), 1/d, and the pair
 // record of particle i, with Pi/d^2 in the reference's arithmetic
 // (src/kernel_forces.cu:64-88, 547-558).
-template <class R>
+template <class R, int DIM>
 __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Particles<R> &S,
-                                               const Derived<R> &D, PairRec<R> *rec, int i,
+                                               const Derived<R> &D, PairRec<R, DIM> *rec, int i,
                                                R d) {
   const R p = (R)P.csq * (S.mass[i] * d - (R)P.rho0);
   const R dinv = R(1) / d;
@@ -1025,34 +1122,44 @@ __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Part
   D.press[i] = p;
   D.dinv[i] = dinv;
   const int t = S.type[i];
-  const bool d3 = P.dim == 3;
+  constexpr bool d3 = DIM == 3;
   R c[6];
+#pragma unroll
   for (int k = 0; k < 6; ++k) c[k] = (d3 || k < 3) ? S.c[k][i] : R(0);
-  PairRec<R> r;
-  r.x = S.x[i];
-  r.y = S.y[i];
-  r.z = d3 ? S.z[i] : 0.0;
-  r.v[0] = S.v[0][i];
-  r.v[1] = S.v[1][i];
-  r.v[2] = d3 ? S.v[2][i] : R(0);
+  PairRec<R, DIM> r;
+  if constexpr (kFixRec<R, DIM>) {
+    const uint4 X = S.pq[i];
+    r.X[0] = X.x;
+    r.X[1] = X.y;
+    if constexpr (d3) r.X[2] = X.z;
+  } else {
+    r.x[0] = S.x[i];
+    r.x[1] = S.y[i];
+    if constexpr (d3) r.x[2] = S.z[i];
+  }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) r.v[a] = S.v[a][i];
   if (t == 0) {
     r.w = dinv;
     r.q[0] = (p - nkT * c[0]) * dinv2;
     r.q[1] = (-nkT * c[1]) * dinv2;
     r.q[2] = (p - nkT * c[2]) * dinv2;
-    r.q[3] = d3 ? (-nkT * c[3]) * dinv2 : R(0);
-    r.q[4] = d3 ? (-nkT * c[4]) * dinv2 : R(0);
-    r.q[5] = d3 ? (p - nkT * c[5]) * dinv2 : R(0);
+    if constexpr (d3) {
+      r.q[3] = (-nkT * c[3]) * dinv2;
+      r.q[4] = (-nkT * c[4]) * dinv2;
+      r.q[5] = (p - nkT * c[5]) * dinv2;
+    }
   } else {
     r.w = -(R)t;
-    for (int k = 0; k < 6; ++k) r.q[k] = c[k];
+#pragma unroll
+    for (int k = 0; k < (d3 ? 6 : 3); ++k) r.q[k] = c[k];
   }
   rec[i] = r;
 }
 
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable T, Particles<R> S, Derived<R> D,
-                                                    PairRec<R> *rec, const int *__restrict__ csg,
+                                                    PairRec<R, DIM> *rec, const int *__restrict__ csg,
                                                     NList L, int write_dens, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
@@ -1318,7 +1425,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
   L.count[i] = cnt;
   if (write_dens) {
     D.dens[i] = d;
-    pressure_terms<R>(P, S, D, rec, i, d);
+    pressure_terms<R, DIM>(P, S, D, rec, i, d);
   }
 }
 
@@ -1409,13 +1516,13 @@ __global__ __launch_bounds__(kBlock) void k_pack_pos(SimTable T, Particles<R> S)
   store_pos(S, T.P[sim_of(T, i)], i, S.x[i], S.y[i], S.z[i]);
 }
 
-template <class R>
+template <class R, int DIM>
 __global__ __launch_bounds__(kBlock) void k_pressure(SimTable T, Particles<R> S, Derived<R> D,
-                                                     PairRec<R> *rec) {
+                                                     PairRec<R, DIM> *rec) {
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int i = T.blk_first[b] + threadIdx.x;
   if (i >= T.pbeg[sim + 1]) return;
-  pressure_terms<R>(T.P[sim], S, D, rec, i, D.dens[i]);
+  pressure_terms<R, DIM>(T.P[sim], S, D, rec, i, D.dens[i]);
 }
 
 // Pi / d^2 with Pi = p I - nkT c, components xx xy yy xz yz zz
@@ -1457,30 +1564,26 @@ struct PairAcc {
   R f[3];     // force sum
 };
 
-// Everything about particle i that the pair body reads.
-template <class R>
-struct PairSelf {
-  double x, y, z;
-  int t;
-  R v[3];
-  R dinv, p;
-  R pd[6];
-};
 
 // One neighbour j of i: velocity-gradient and force contributions with the
 // reference's pair selection, Morris velocities and Pi/d^2 substitutions
 // (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
-__device__ __forceinline__ void pair_body(const PairK<R> &K, const PairRec<R> &rj,
+__device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<R> &rj,
                                           const Bodies &B, const PairSelf<R> &me,
                                           PairAcc<R> &acc) {
   const int tj = rj.w < R(0) ? (int)(-rj.w) : 0;
   const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
   const bool on_f = DO_F && (me.t == 0 || tj == 0);
   if (!on_gv && !on_f) return;
-  const double xj = rj.x, yj = rj.y, zj = rj.z;
   R r[3];
-  pair_sep<R, DIM, WALL>(K.X, me.x, me.y, me.z, xj, yj, zj, r);
+  if constexpr (kFixRec<R, DIM>) {
+    const uint4 a = make_uint4(me.X[0], me.X[1], me.X[2], 0u);
+    const uint4 b = make_uint4(rj.X[0], rj.X[1], rj.X[2], 0u);
+    fix_sep<DIM>(K.u, a, b, r);
+  } else {
+    pair_sep<R, DIM, WALL>(K.X, me.x, me.y, me.z, rj.x[0], rj.x[1], rj.x[2], r);
+  }
   const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
   R rr, ri;
   pair_root(r2, rr, ri);
@@ -1488,7 +1591,14 @@ __device__ __forceinline__ void pair_body(const PairK<R> &K, const PairRec<R> &r
   const R gw = sph_gradW<R>(rr, K.rcut_inv, K.c_gradw);
   const R vj[3] = {rj.v[0], rj.v[1], rj.v[2]};
   R v[3];
-  morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, xj, yj, zj, me.v, vj, r, v);
+  if constexpr (kFixRec<R, DIM>) {
+    // (the fixed-point coordinates mapped back; used only by wall and colloid pairs)
+    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.pos(K, 0), me.pos(K, 1), 0.0, pos_of(K, rj, 0),
+                             pos_of(K, rj, 1), 0.0, me.v, vj, r, v);
+  } else {
+    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, rj.x[0], rj.x[1], rj.x[2], me.v,
+                             vj, r, v);
+  }
   if (on_gv) {
     // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
     const R coef = -me.dinv * gw;
@@ -1550,11 +1660,18 @@ __device__ __forceinline__ void pair_body(const PairK<R> &K, const PairRec<R> &r
 
 template <class R, int DIM>
 __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles<R> &S,
-                                          const Derived<R> &D, const PairRec<R> *rec, int i,
-                                          bool forces, PairSelf<R> &me) {
-  me.x = S.x[i];
-  me.y = S.y[i];
-  me.z = DIM == 3 ? S.z[i] : 0.0;
+                                          const Derived<R> &D, const PairRec<R, DIM> *rec,
+                                          int i, bool forces, PairSelf<R> &me) {
+  if constexpr (kFixRec<R, DIM>) {
+    const uint4 X = S.pq[i];
+    me.X[0] = X.x;
+    me.X[1] = X.y;
+    me.X[2] = X.z;
+  } else {
+    me.x = S.x[i];
+    me.y = S.y[i];
+    me.z = DIM == 3 ? S.z[i] : 0.0;
+  }
   me.t = S.type[i];
   me.v[0] = S.v[0][i];
   me.v[1] = S.v[1][i];
@@ -1564,7 +1681,7 @@ __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles
   // Pi/d^2 of a fluid particle (only fluid particles use their own)
   const bool own = forces && me.t == 0;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) me.pd[q] = own ? rec[i].q[q] : R(0);
+  for (int q = 0; q < 6; ++q) me.pd[q] = own && (DIM == 3 || q < 3) ? rec[i].q[q] : R(0);
 }
 
 // Writes the results of particle i: force (+ external force), gradient tensor
@@ -1575,7 +1692,7 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
                                             PairAcc<R> &acc, int store_gv, int write_dcdt,
                                             int with_ext) {
   if (DO_F) {
-    if (with_ext && me.t == 0) ext_force<R, DIM>(P, S.mass[i], me.y, acc.f);
+    if (with_ext && me.t == 0) ext_force<R, DIM>(P, S.mass[i], S.y[i], acc.f);
     D.f[0][i] = acc.f[0];
     D.f[1][i] = acc.f[1];
     D.f[2][i] = DIM == 3 ? acc.f[2] : R(0);
@@ -1634,7 +1751,8 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_PAIRS_WAVES_F64) void k_pairs(
-    SimTable T, Particles<R> S, Derived<R> D, Bodies Ball, const PairRec<R> *__restrict__ rec,
+    SimTable T, Particles<R> S, Derived<R> D, Bodies Ball,
+    const PairRec<R, DIM> *__restrict__ rec,
     NList L, int store_gv, int write_dcdt, int with_ext, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const int iend = T.pbeg[sim + 1];
@@ -1657,27 +1775,31 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
 #pragma unroll
     for (int b = 0; b < 3; ++b) acc.g[a][b] = R(0);
   }
-  const PairK<R> K(P);
+  const PairK<R, DIM> K(P);
   const bool vok = VL.idx && live && VL.count[i] <= VL.cap;
   if (VL.idx) {
     // The Verlet entries the density pass marked, in list order, all lanes of
     // the wave stepping through entry e together (a lane past its list or not
     // on the Verlet path has no hits).  Index and mask loads are coalesced.
-    // Records move in two stages: (1) the four lanes of quad a load the record
-    // lane 4a + k needs as 16-byte chunks, lane 4a + b taking chunk b (FP32:
-    // one instruction per record; FP64: two, chunks b and 4 + b), so an
-    // instruction touches one line per quad instead of one per lane; (2) the
-    // chunks go through LDS to the lane that needs them.  The quad loads of
-    // entry e + 1 are issued before entry e is evaluated.  A lane without a hit
-    // asks for an offset past the simulation's records, which the buffer
-    // returns as zeros without a memory access.
+    // Records move in two stages: (1) the G lanes of group a (G = 4, or 2 for
+    // the two-chunk 2D FP32 record) load the record lane Ga + k needs as
+    // 16-byte chunks, lane Ga + b taking chunk b (the 128-byte FP64 3D record
+    // takes two instructions, chunks b and 4 + b), so an instruction touches
+    // one line per group instead of one per lane; (2) the chunks go through LDS
+    // to the lane that needs them.  The group loads of entry e + 1 are issued
+    // before entry e is evaluated.  A lane without a hit asks for an offset past
+    // the simulation's records, which the buffer returns as zeros without a
+    // memory access.
     typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    constexpr int kChunks = sizeof(PairRec<R>) / 16;  // 4 (FP32) or 8 (FP64, last is padding)
-    constexpr int kLoads = kChunks;                    // quad instructions per entry
+    using Rec = PairRec<R, DIM>;
+    constexpr int kChunks = sizeof(Rec) / 16;          // 2, 4 or 8 (FP64 3D: last is padding)
+    constexpr int kG = kChunks < 4 ? kChunks : 4;      // lanes per group
+    constexpr int kLoads = kChunks;                    // group instructions per entry
+    constexpr int kPer = kChunks / kG;                 // instructions per record
     constexpr int kSlot = kChunks + 1;  // padded LDS slots: conflict-free rows
     __shared__ u4 stage[kBlock / 64][64][kSlot];
     u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
-    const int lane = threadIdx.x & 63, qb = lane & 3, qa = lane & ~3;
+    const int lane = threadIdx.x & 63, qb = lane & (kG - 1), qa = lane & ~(kG - 1);
     const int vc = vok ? VL.count[i] : 0;
     int vmax = vc;
 #pragma unroll
@@ -1688,7 +1810,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const RecBuf rb = rec_buf(rec, T, sim);
     const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
     const unsigned oob =
-        (unsigned)(__builtin_amdgcn_readfirstlane(iend) - pb) * (unsigned)sizeof(PairRec<R>);
+        (unsigned)(__builtin_amdgcn_readfirstlane(iend) - pb) * (unsigned)sizeof(Rec);
     const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
     // q: indices of the current group of four, rotated so that q.x is the
     // next entry; qn: the following group.  m: hit bits of the current 32
@@ -1699,16 +1821,20 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     unsigned m = vm[0];
     unsigned mn = vm[(size_t)min(1, lastw) * N];
     if (vc < 32) m &= (1u << vc) - 1u;  // (vc = 0: no hits)
-    // instruction k of a quad load: the record of lane qa + src(k), chunk chk(k)
-    auto src = [](int k) { return kLoads == 4 ? k : k >> 1; };
-    auto chk = [&](int k) { return kLoads == 4 ? qb : qb + 4 * (k & 1); };
+    // instruction k of a group load: the record of lane qa + src(k), chunk chk(k)
+    auto src = [](int k) { return k / kPer; };
+    auto chk = [&](int k) { return qb + kG * (k % kPer); };
     auto quad_load = [&](unsigned off, u4(&v)[kLoads]) {
       const int mine = (int)off;
 #pragma unroll
       for (int k = 0; k < kLoads; ++k) {
-        const int o = src(k) == 0 ? quad_bcast<0>(mine)
-                    : src(k) == 1 ? quad_bcast<1>(mine)
-                    : src(k) == 2 ? quad_bcast<2>(mine) : quad_bcast<3>(mine);
+        int o;
+        if constexpr (kG == 4)
+          o = src(k) == 0 ? quad_bcast<0>(mine)
+            : src(k) == 1 ? quad_bcast<1>(mine)
+            : src(k) == 2 ? quad_bcast<2>(mine) : quad_bcast<3>(mine);
+        else  // lane pairs: lanes 2a, 2a + 1 read lane 2a + k
+          o = src(k) == 0 ? quad_pick<0xA0>(mine) : quad_pick<0xF5>(mine);
         v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, o + 16 * chk(k), 0, 0);
       }
     };
@@ -1727,23 +1853,19 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
         const int left = vc - (e + 1);
         if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
       }
-      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, nxt);
+      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(Rec) : oob, nxt);
 #pragma unroll
       for (int k = 0; k < kLoads; ++k) st[qa + src(k)][chk(k)] = cur[k];
       __builtin_amdgcn_wave_barrier();
-      u4 mine[kRecVecs<R>];
+      u4 mine[kRecVecs<R, DIM>];
 #pragma unroll
-      for (int k = 0; k < kRecVecs<R>; ++k) mine[k] = st[lane][k];
+      for (int k = 0; k < kRecVecs<R, DIM>; ++k) mine[k] = st[lane][k];
       __builtin_amdgcn_wave_barrier();
-      if (hit) {
-        PairRec<R> rj;
-        __builtin_memcpy(&rj, mine, sizeof(mine));
-        pair_body<R, DIM, WALL, DO_GV, DO_F>(K, rj, B, me, acc);
-      }
+      if (hit) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, unpack_chunks<R, DIM>(mine), B, me, acc);
     };
     // two entries per trip with the chunk buffers swapping roles (no copies)
     u4 ca[kLoads], cb[kLoads];
-    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(PairRec<R>) : oob, ca);
+    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(Rec) : oob, ca);
 #pragma unroll 1
     for (int e = 0; e < vmax; e += 2) {
       entry(e, ca, cb);
@@ -1766,7 +1888,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const int m = min(4, cnt - k);
 #pragma unroll 1
     for (int u = 0; u < m; ++u) {
-      pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R>(rbn, q.x - pbn), B, me, acc);
+      pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R, DIM>(rbn, q.x - pbn), B, me, acc);
       q = make_int4(q.y, q.z, q.w, 0);
     }
   }
@@ -1790,7 +1912,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> S,
                                                        Derived<R> D, Bodies Ball,
-                                                       const PairRec<R> *__restrict__ rec, NList L,
+                                                       const PairRec<R, DIM> *__restrict__ rec,
+                                                       NList L,
                                                        const int *__restrict__ csg, int store_gv,
                                                        int write_dcdt, int with_ext) {
   const int n = *L.qlen;
@@ -1810,11 +1933,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
       for (int b = 0; b < 3; ++b) acc.g[a][b] = D.gv[a * 3 + b][i];
     }
     const R rcutsq = (R)P.rcutsq;
-    const PairK<R> K(P);
+    const PairK<R, DIM> K(P);
     const int skip = L.kmax;
     int hit = 0;
     int cx, cy, cz;
-    cell_coords<DIM, WALL>(P, me.x, me.y, me.z, cx, cy, cz);
+    cell_coords<DIM, WALL>(P, S.x[i], S.y[i], DIM == 3 ? S.z[i] : 0.0, cx, cy, cz);
     CandSelf<R, DIM> cme;
     cme.init(S, i, P);
     for_each_strip<DIM, WALL>(P, cs, cx, cy, cz, [&](int k0, int k1, double sx, double sy,
@@ -2320,10 +2443,10 @@ class DeviceImpl final : public Device {
         err = "all simulations of a batch must share dim and wall";
         return SPH_ERR_ARG;
       }
-      if ((unsigned long long)p.N * sizeof(PairRec<R>) > 0xFFFFFFFFull) {
+      if ((unsigned long long)p.N * sizeof(PairRec<R, 3>) > 0xFFFFFFFFull) {
         // record loads address a simulation's records with 32-bit offsets
         err = "a simulation of " + std::to_string(p.N) + " particles exceeds the " +
-              std::to_string(0xFFFFFFFFull / sizeof(PairRec<R>)) +
+              std::to_string(0xFFFFFFFFull / sizeof(PairRec<R, 3>)) +
               "-particle limit of this precision form";
         return SPH_ERR_ARG;
       }
@@ -2427,7 +2550,7 @@ class DeviceImpl final : public Device {
     HIPCHK(nl_qlen_.alloc(1));
     HIPCHK(hipMemset(nl_qlen_.p, 0, sizeof(int)));
     HIPCHK(dinv_.alloc(N_));
-    HIPCHK(rec_.alloc(N_));
+    HIPCHK(rec_.alloc((size_t)N_ * sizeof(PairRec<R, 3>)));  // the larger record
     HIPCHK(ke_part_.alloc((size_t)kKeBlocks * S)); HIPCHK(ke_out_.alloc(S));
     HIPCHK(obs_part_.alloc((size_t)3 * kKeBlocks * S)); HIPCHK(obs_out_.alloc(6 * S));
     for (int k = 0; k < C_NUM; ++k) HIPCHK(coll_[k].alloc(ncoll_));
@@ -2660,8 +2783,8 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_density<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(),
-                                                              rec_.p, cs_.p, nlist(), write_dens,
-                                                              vlist());
+                                                              recs<DIM>(), cs_.p, nlist(),
+                                                              write_dens, vlist());
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
@@ -2675,7 +2798,11 @@ class DeviceImpl final : public Device {
 
   // calculate_pressure: pressure, 1/d and the pair records from the current densities.
   int pressure(std::string &err) override {
-    k_pressure<R><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(), rec_.p);
+    dispatch3([&](auto dimc, auto) {
+      constexpr int DIM = decltype(dimc)::value;
+      k_pressure<R, DIM><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(),
+                                                         recs<DIM>());
+    });
     HIPCHK(hipGetLastError());
     rec_valid_ = true;
     return SPH_OK;
@@ -2695,10 +2822,10 @@ class DeviceImpl final : public Device {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<nblk_, block_, 0, stream_>>>(
-          table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), store_gv, write_dcdt,
-          with_ext, vlist());
+          table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), store_gv,
+          write_dcdt, with_ext, vlist());
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
-          table(), parts(cur_), derived(), bodies(), rec_.p, nlist(), cs_.p, store_gv,
+          table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), cs_.p, store_gv,
           write_dcdt, with_ext);
     });
     HIPCHK(hipGetLastError());
@@ -3303,7 +3430,9 @@ class DeviceImpl final : public Device {
   DevBuf<int> newpos_;  // old sorted position -> new, of the last stand-alone reorder
   DevBuf<R> ptmp_;
   DevBuf<R> dens_, press_, dinv_, f_[3], dcdt_[6], gv_[9];
-  DevBuf<PairRec<R>> rec_;
+  DevBuf<unsigned char> rec_;  // pair records, PairRec<R, dim> each
+  template <int DIM>
+  PairRec<R, DIM> *recs() { return reinterpret_cast<PairRec<R, DIM> *>(rec_.p); }
   bool rec_valid_ = false;
   DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, tsum_, flag_;
   DevBuf<int4> nl_idx_;
