@@ -1814,12 +1814,14 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
     // q: indices of the current group of four, rotated so that q.x is the
     // next entry; qn: the following group.  m: hit bits of the current 32
-    // entries, bit 0 the next entry (masked to zero past the list); mn: the
-    // following 32 (unmasked until used).
+    // entries, bit 0 the next entry (masked to zero past the list); mn, mn2: the
+    // following two words (unmasked until used; two ahead so that the word
+    // taken over is never one still being loaded).
     int4 q = vl[0];
     int4 qn = vl[(size_t)min(1, lastg) * N];
     unsigned m = vm[0];
     unsigned mn = vm[(size_t)min(1, lastw) * N];
+    unsigned mn2 = vm[(size_t)min(2, lastw) * N];
     if (vc < 32) m &= (1u << vc) - 1u;  // (vc = 0: no hits)
     // instruction k of a group load: the record of lane qa + src(k), chunk chk(k)
     auto src = [](int k) { return k / kPer; };
@@ -1849,7 +1851,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
       }
       if (((e + 1) & 31) == 0) {
         m = mn;
-        mn = vm[(size_t)min((e + 33) >> 5, lastw) * N];
+        mn = mn2;
+        mn2 = vm[(size_t)min((e + 65) >> 5, lastw) * N];
         const int left = vc - (e + 1);
         if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
       }
@@ -1866,10 +1869,14 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     // two entries per trip with the chunk buffers swapping roles (no copies)
     u4 ca[kLoads], cb[kLoads];
     quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(Rec) : oob, ca);
+    // (Always both entries of a trip: with a trip that could end after the first,
+    // the loads of the first would still be pending at the loop head and every
+    // trip would wait for them before issuing its own.  Entry vmax, evaluated
+    // when vmax is odd, is past every lane's list: no hit, no record access.)
 #pragma unroll 1
     for (int e = 0; e < vmax; e += 2) {
       entry(e, ca, cb);
-      if (e + 1 < vmax) entry(e + 1, cb, ca);
+      entry(e + 1, cb, ca);
     }
     if (!live) return;
     if (vok) {
