@@ -152,6 +152,11 @@ struct VerletCtl {
   float4 *ref;         // [N] positions at the last rebuild
   float half_skin_frac;  // (skin / 2) / rcut, slightly reduced for rounding
   int on;              // displacement checks active
+  // [1] some simulation of the batch rebuilds in the step this flag belongs to:
+  // raised with rebuild[s]; a step's hash, scan and scatter exit at once while
+  // it is 0.  One flag per buffer parity (a step flips the parity): the step of
+  // parity p reads flag p and clears flag p ^ 1 for the next step.
+  int *any;
 };
 
 struct VList {
@@ -780,7 +785,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int *cell, int *slot,
-                                                 int *count, int *flag) {
+                                                 int *count, int *flag, const int *any) {
+  if (any && !*any) return;  // no simulation rebuilds this step
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
@@ -865,7 +871,9 @@ __device__ __forceinline__ int block_scan8(const int (&v)[8], int *total) {
   return wsum[wid] + incl - local;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *tsum, int ncells) {
+__global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *tsum, int ncells,
+                                                     const int *any) {
+  if (any && !*any) return;
   const int c0 = blockIdx.x * kScanTile + threadIdx.x * 8;
   int v[8];
 #pragma unroll
@@ -886,7 +894,8 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *t
 // One workgroup: tile totals -> exclusive tile offsets (at most kScanTile
 // tiles), and the grand total into cs[ncells].
 __global__ __launch_bounds__(1024) void k_scan_tile_sums(int *tsum, int ntiles, int *cs,
-                                                         int ncells) {
+                                                         int ncells, const int *any) {
+  if (any && !*any) return;
   const int c0 = threadIdx.x * 8;
   int v[8];
 #pragma unroll
@@ -901,7 +910,9 @@ __global__ __launch_bounds__(1024) void k_scan_tile_sums(int *tsum, int ntiles, 
   if (threadIdx.x == 0) cs[ncells] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, int ncells) {
+__global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, int ncells,
+                                                     const int *any) {
+  if (any && !*any) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncells && c >= kScanTile) cs[c] += tsum[c / kScanTile];
 }
@@ -911,7 +922,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__res
                                                     const int *__restrict__ slot,
                                                     const int *__restrict__ cs,
                                                     const int *__restrict__ id, int *stage_src,
-                                                    int *stage_id, const int *rebuild) {
+                                                    int *stage_id, const int *rebuild,
+                                                    const int *any) {
+  if (any && !*any) return;
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int k = T.blk_first[b] + threadIdx.x;
   if (k >= T.pbeg[sim + 1] || (rebuild && !rebuild[sim])) return;
@@ -931,7 +944,8 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     SimTable T, Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
     const int *__restrict__ cs, const int *__restrict__ stage_src,
     const int *__restrict__ stage_id, int *rank, int *sorted_cell, int nc,
-    const int *rebuild, int *newpos) {
+    const int *rebuild, int *newpos, int *any_next) {
+  if (any_next && blockIdx.x == 0 && threadIdx.x == 0) *any_next = 0;  // the next step's flag
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int p = T.blk_first[b] + threadIdx.x;
   if (p >= T.pbeg[sim + 1]) return;
@@ -1500,6 +1514,7 @@ __global__ void k_verlet_done(VerletCtl V, int nsim) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsim) return;
   V.rebuild[s] = V.overflow[s] ? 1 : 0;
+  if (V.overflow[s]) atomicOr(V.any, 1);  // (the next step's flag)
   V.overflow[s] = 0;
 }
 
@@ -1593,8 +1608,9 @@ __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<
   R v[3];
   if constexpr (kFixRec<R, DIM>) {
     // (the fixed-point coordinates mapped back; used only by wall and colloid pairs)
-    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.pos(K, 0), me.pos(K, 1), 0.0, pos_of(K, rj, 0),
-                             pos_of(K, rj, 1), 0.0, me.v, vj, r, v);
+    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.pos(K, 0), me.pos(K, 1),
+                             DIM == 3 ? me.pos(K, 2) : 0.0, pos_of(K, rj, 0), pos_of(K, rj, 1),
+                             DIM == 3 ? pos_of(K, rj, 2) : 0.0, me.v, vj, r, v);
   } else {
     morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, rj.x[0], rj.x[1], rj.x[2], me.v,
                              vj, r, v);
@@ -2079,11 +2095,11 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
 // ---------------------------------------------------------------------------
 // integrators
 // ---------------------------------------------------------------------------
-// Raises rebuild[sim] when the particle at sorted position i has moved more than
-// skin / 2 from its position at the last rebuild (minimum image; float).
-__device__ __forceinline__ void displacement_check(const VerletCtl &V, const sph_params_t &P,
-                                                   int sim, int i, double x, double y, double z) {
-  if (!V.on) return;
+// True when the particle at sorted position i has moved more than skin / 2 from
+// its position at the last rebuild (minimum image; float), or its position is
+// not finite (the rebuild's hash then flags its simulation).
+__device__ __forceinline__ bool displaced(const VerletCtl &V, const sph_params_t &P, int i,
+                                          double x, double y, double z) {
   const float4 r = V.ref[i];
   float dx = (float)x - r.x, dy = (float)y - r.y, dz = P.dim == 3 ? (float)z - r.z : 0.0f;
   const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
@@ -2091,7 +2107,15 @@ __device__ __forceinline__ void displacement_check(const VerletCtl &V, const sph
   if (!P.wall) dy -= Ly * rintf(dy / Ly);
   if (P.dim == 3) dz -= Lz * rintf(dz / Lz);
   const float lim = V.half_skin_frac * (float)P.rcut;
-  if (dx * dx + dy * dy + dz * dz > lim * lim) atomicOr(&V.rebuild[sim], 1);
+  return !(dx * dx + dy * dy + dz * dz <= lim * lim);
+}
+
+// Raises rebuild[sim] and the batch flag for a displaced particle.
+__device__ __forceinline__ void displacement_check(const VerletCtl &V, const sph_params_t &P,
+                                                   int sim, int i, double x, double y, double z) {
+  if (!V.on || !displaced(V, P, i, x, y, z)) return;
+  atomicOr(&V.rebuild[sim], 1);
+  atomicOr(V.any, 1);
 }
 
 template <class R, int DIM, bool WALL>
@@ -2134,7 +2158,16 @@ __global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Deri
   S.y[i] = yi;
   if (DIM == 3) S.z[i] = zi;
   store_pos(S, P, i, xi, yi, zi);
-  displacement_check(V, P, sim, i, xi, yi, zi);
+  if (V.on) {
+    // one flag update per wavefront (its lanes belong to one simulation): a
+    // moving wall passes the limit with all its particles in the same step
+    const bool far = displaced(V, P, i, xi, yi, zi);
+    const unsigned long long w = __ballot(far);
+    if (far && (int)(threadIdx.x & 63) == __ffsll((long long)w) - 1) {
+      atomicOr(&V.rebuild[sim], 1);
+      atomicOr(V.any, 1);
+    }
+  }
 }
 
 template <class R, int DIM>
@@ -2550,6 +2583,8 @@ class DeviceImpl final : public Device {
       HIPCHK(voverflow_.alloc(S));
       HIPCHK(hipMemset(vrebuild_.p, 0, S * sizeof(int)));
       HIPCHK(hipMemset(voverflow_.p, 0, S * sizeof(int)));
+      HIPCHK(any_.alloc(2));
+      HIPCHK(hipMemset(any_.p, 0, 2 * sizeof(int)));
     }
     verlet_valid_ = verlet_dirty_ = in_verlet_ = false;
     HIPCHK(nl_count_.alloc(N_));
@@ -2735,17 +2770,22 @@ class DeviceImpl final : public Device {
   int reorder(const int *gate, bool carry, std::string &err) {
     Mark m(this, PH_NS);
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
+    // With a gate (a Verlet step), the batch flag of this step's parity lets
+    // hash, scan and scatter exit at once when no simulation rebuilds; the
+    // gather clears the flag of the next step's parity.
+    int *any = gate ? any_.p + cur_ : nullptr, *any_next = gate ? any_.p + (cur_ ^ 1) : nullptr;
     k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
-                                          flag_.p);
+                                          flag_.p, any);
     const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
-    k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_);
-    k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_);
-    if (ntiles > 1) k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_);
+    k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_, any);
+    k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_, any);
+    if (ntiles > 1)
+      k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_, any);
     k_scatter<<<nblk_, block_, 0, stream_>>>(table(), cell_.p, slot_.p, cs_.p, S.id,
-                                             stage_src_.p, stage_id_.p, gate);
+                                             stage_src_.p, stage_id_.p, gate, any);
     k_rank_gather<R><<<nblk_, block_, 0, stream_>>>(table(), S, T, cell_.p, cs_.p, stage_src_.p,
                                                     stage_id_.p, rank_.p, sorted_cell_.p, nc_,
-                                                    gate, carry ? newpos_.p : nullptr);
+                                                    gate, carry ? newpos_.p : nullptr, any_next);
     HIPCHK(hipGetLastError());
     if (carry) {
       // A stand-alone neighbour search: the fields computed before it (density,
@@ -2972,6 +3012,7 @@ class DeviceImpl final : public Device {
     if (verlet_ok_ && nsteps > 0) {
       if (!verlet_valid_) {  // lists of a different order or positions: rebuild all
         k_fill<<<blocks(nsim()), kBlock, 0, stream_>>>(vrebuild_.p, nsim(), 1);
+        k_fill<<<1, kBlock, 0, stream_>>>(any_.p, 2, 1);
         HIPCHK(hipGetLastError());
       }
       verlet_valid_ = true;
@@ -3347,7 +3388,8 @@ class DeviceImpl final : public Device {
     return v;
   }
   VerletCtl vctl() {
-    VerletCtl v{vrebuild_.p, voverflow_.p, ref4_.p, 0.49f * skin_frac_, in_verlet_ ? 1 : 0};
+    VerletCtl v{vrebuild_.p, voverflow_.p, ref4_.p, 0.49f * skin_frac_, in_verlet_ ? 1 : 0,
+                any_.p ? any_.p + cur_ : nullptr};
     return v;
   }
 
@@ -3421,6 +3463,7 @@ class DeviceImpl final : public Device {
   DevBuf<int4> vl_idx_;
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
+  DevBuf<int> any_;  // [2] batch rebuild flag per buffer parity (VerletCtl::any)
   DevBuf<float4> ref4_;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
   hipStream_t stream_ = nullptr;
