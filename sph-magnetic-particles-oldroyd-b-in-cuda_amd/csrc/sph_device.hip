@@ -152,10 +152,10 @@ struct VerletCtl {
   float4 *ref;         // [N] positions at the last rebuild
   float half_skin_frac;  // (skin / 2) / rcut, slightly reduced for rounding
   int on;              // displacement checks active
-  // [1] some simulation of the batch rebuilds in the step this flag belongs to:
-  // raised with rebuild[s]; a step's hash, scan and scatter exit at once while
-  // it is 0.  One flag per buffer parity (a step flips the parity): the step of
-  // parity p reads flag p and clears flag p ^ 1 for the next step.
+  // [1] some simulation of the batch rebuilds this step: raised with rebuild[s];
+  // the step's hash, scan and scatter exit at once while it is 0, and the
+  // step's copy-back clears it for the next step (after which an overflowing
+  // build raises it again).
   int *any;
 };
 
@@ -937,19 +937,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__res
 
 // Final position = cell start + number of particles of the same cell with a
 // smaller ID, i.e. the stable (cell, ID) order.  Gathers the moving state.
-// A simulation that is not rebuilt keeps its order: its state is copied
-// position for position into the other buffer set (the step always flips).
+// A simulation that is not rebuilt keeps its order: with copy_others its state
+// is copied position for position into the other buffer set (a full reorder
+// flips the sets); without, it is left alone (a Verlet step copies the rebuilt
+// simulations back instead, k_copy_back).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
     SimTable T, Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
     const int *__restrict__ cs, const int *__restrict__ stage_src,
     const int *__restrict__ stage_id, int *rank, int *sorted_cell, int nc,
-    const int *rebuild, int *newpos, int *any_next) {
-  if (any_next && blockIdx.x == 0 && threadIdx.x == 0) *any_next = 0;  // the next step's flag
+    const int *rebuild, int *newpos, int copy_others) {
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int p = T.blk_first[b] + threadIdx.x;
   if (p >= T.pbeg[sim + 1]) return;
   if (rebuild && !rebuild[sim]) {
+    if (!copy_others) return;
     dst.x[p] = src.x[p];
     dst.y[p] = src.y[p];
     dst.z[p] = src.z[p];
@@ -982,6 +984,30 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   rank[myid] = d;
   sorted_cell[d] = c;
   if (newpos) newpos[k] = d;
+}
+
+// A Verlet step reorders in place: the rebuilt simulations' state, gathered into
+// the other buffer set, is copied back (x v c m type ID; the packed position
+// copies were written in the new order by the gather).  Simulations that keep
+// their order are not touched.  Block 0 clears the batch rebuild flag: every
+// reader of it in this step has run.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_copy_back(SimTable T, Particles<R> src,
+                                                      Particles<R> dst, int nc,
+                                                      const int *rebuild, int *any) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *any = 0;
+  const int b = blockIdx.x, sim = T.blk_sim[b];
+  const int p = T.blk_first[b] + threadIdx.x;
+  if (p >= T.pbeg[sim + 1] || !rebuild[sim]) return;
+  dst.x[p] = src.x[p];
+  dst.y[p] = src.y[p];
+  dst.z[p] = src.z[p];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) dst.v[a][p] = src.v[a][p];
+  for (int a = 0; a < nc; ++a) dst.c[a][p] = src.c[a][p];
+  dst.mass[p] = src.mass[p];
+  dst.type[p] = src.type[p];
+  dst.id[p] = src.id[p];
 }
 
 // tmp[newpos[k]] = a[k]: a per-particle field carried into the new order.
@@ -2583,8 +2609,8 @@ class DeviceImpl final : public Device {
       HIPCHK(voverflow_.alloc(S));
       HIPCHK(hipMemset(vrebuild_.p, 0, S * sizeof(int)));
       HIPCHK(hipMemset(voverflow_.p, 0, S * sizeof(int)));
-      HIPCHK(any_.alloc(2));
-      HIPCHK(hipMemset(any_.p, 0, 2 * sizeof(int)));
+      HIPCHK(any_.alloc(1));
+      HIPCHK(hipMemset(any_.p, 0, sizeof(int)));
     }
     verlet_valid_ = verlet_dirty_ = in_verlet_ = false;
     HIPCHK(nl_count_.alloc(N_));
@@ -2764,16 +2790,17 @@ class DeviceImpl final : public Device {
     return reorder(nullptr, true, err);
   }
 
-  // Hash, scan, reorder.  With `gate`, simulations whose gate[s] is 0 keep
-  // their order (copied into the other buffer set).  `carry` moves the derived
-  // fields along (not needed inside a step, which recomputes them).
+  // Hash, scan, reorder.  Without `gate` every simulation is reordered into the
+  // other buffer set, which becomes the current one.  With `gate` (a Verlet
+  // step) only simulations whose gate[s] is set are reordered, in place
+  // (gathered into the other set and copied back), and the batch rebuild flag
+  // lets hash, scan and scatter exit at once when there are none.  `carry`
+  // moves the derived fields along (not needed inside a step, which recomputes
+  // them).
   int reorder(const int *gate, bool carry, std::string &err) {
     Mark m(this, PH_NS);
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
-    // With a gate (a Verlet step), the batch flag of this step's parity lets
-    // hash, scan and scatter exit at once when no simulation rebuilds; the
-    // gather clears the flag of the next step's parity.
-    int *any = gate ? any_.p + cur_ : nullptr, *any_next = gate ? any_.p + (cur_ ^ 1) : nullptr;
+    int *any = gate ? any_.p : nullptr;
     k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
                                           flag_.p, any);
     const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
@@ -2785,7 +2812,10 @@ class DeviceImpl final : public Device {
                                              stage_src_.p, stage_id_.p, gate, any);
     k_rank_gather<R><<<nblk_, block_, 0, stream_>>>(table(), S, T, cell_.p, cs_.p, stage_src_.p,
                                                     stage_id_.p, rank_.p, sorted_cell_.p, nc_,
-                                                    gate, carry ? newpos_.p : nullptr, any_next);
+                                                    gate, carry ? newpos_.p : nullptr,
+                                                    gate ? 0 : 1);
+    if (gate)
+      k_copy_back<R><<<nblk_, block_, 0, stream_>>>(table(), T, S, nc_, gate, any);
     HIPCHK(hipGetLastError());
     if (carry) {
       // A stand-alone neighbour search: the fields computed before it (density,
@@ -2802,7 +2832,7 @@ class DeviceImpl final : public Device {
       }
       HIPCHK(hipGetLastError());
     }
-    cur_ ^= 1;
+    if (!gate) cur_ ^= 1;
     list_valid_ = false;
     rec_valid_ = false;
     have_cells_ = true;
@@ -3012,7 +3042,7 @@ class DeviceImpl final : public Device {
     if (verlet_ok_ && nsteps > 0) {
       if (!verlet_valid_) {  // lists of a different order or positions: rebuild all
         k_fill<<<blocks(nsim()), kBlock, 0, stream_>>>(vrebuild_.p, nsim(), 1);
-        k_fill<<<1, kBlock, 0, stream_>>>(any_.p, 2, 1);
+        k_fill<<<1, kBlock, 0, stream_>>>(any_.p, 1, 1);
         HIPCHK(hipGetLastError());
       }
       verlet_valid_ = true;
@@ -3025,8 +3055,9 @@ class DeviceImpl final : public Device {
     }
     if (nsteps > 0 && (rc = set_step(first_step - 1, err))) return rc;
     for (int n = 0; n < nsteps; ++n) {
-      // The reorder flips the buffer set, so the captured launches differ by
-      // the parity of the current set: one graph per parity.
+      // A full reorder flips the buffer set (a Verlet step keeps it), so the
+      // captured launches differ by the parity of the current set: one graph
+      // per parity, and the parity the step leaves, recorded at capture.
       const int par = cur_;
       if (!exec_[par]) {
         hipGraph_t g = nullptr;
@@ -3048,10 +3079,11 @@ class DeviceImpl final : public Device {
           err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
           return SPH_ERR_DEVICE;
         }
+        exec_next_[par] = cur_;
         cur_ = par;  // capture recorded the launches without running them
       }
       HIPCHK(hipGraphLaunch(exec_[par], stream_));
-      cur_ = par ^ 1;
+      cur_ = exec_next_[par];
     }
     // state after a step: positions unchanged since the density pass, velocities moved
     list_valid_ = true;
@@ -3389,7 +3421,7 @@ class DeviceImpl final : public Device {
   }
   VerletCtl vctl() {
     VerletCtl v{vrebuild_.p, voverflow_.p, ref4_.p, 0.49f * skin_frac_, in_verlet_ ? 1 : 0,
-                any_.p ? any_.p + cur_ : nullptr};
+                any_.p};
     return v;
   }
 
@@ -3463,11 +3495,12 @@ class DeviceImpl final : public Device {
   DevBuf<int4> vl_idx_;
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
-  DevBuf<int> any_;  // [2] batch rebuild flag per buffer parity (VerletCtl::any)
+  DevBuf<int> any_;  // [1] batch rebuild flag (VerletCtl::any)
   DevBuf<float4> ref4_;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
   hipStream_t stream_ = nullptr;
   hipGraphExec_t exec_[2] = {nullptr, nullptr};
+  int exec_next_[2] = {0, 0};  // the buffer parity each captured step leaves
   std::vector<Ev> events_;
   std::vector<int> id_host_;
 
