@@ -2022,27 +2022,55 @@ __global__ __launch_bounds__(kBlock) void k_ext(SimTable T, Particles<R> S, Deri
 // ---------------------------------------------------------------------------
 // wall and colloid force collection (FP64, deterministic block reductions)
 // ---------------------------------------------------------------------------
+// Wall forces: each wall's list is cut into chunks of kWallChunk particles, one
+// workgroup per chunk (a single workgroup per wall waits out one dependent
+// gather per iteration).  Each workgroup writes its partial sum; the last one of
+// a wall to finish (a counter) adds the partials in chunk order and resets the
+// counter.  The chunking depends only on the wall's particle count, so the sums
+// are deterministic and the same in a batch as alone.
+constexpr int kWallBlock = 256, kWallChunk = 1024;
+
 template <class R>
-__global__ __launch_bounds__(kRedBlock) void k_walls(SimTable T, Derived<R> D, Bodies B,
-                                                     const int *__restrict__ rank) {
-  // block (w, s): wall w of simulation s; the list holds global particle IDs
-  const int w = blockIdx.x, sim = blockIdx.y;
-  const sph_params_t &P = T.P[sim];
+__global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, Bodies B,
+                                                      const int *__restrict__ rank, double *part,
+                                                      unsigned *cnt, int maxch) {
+  // block (chunk, w, s): chunk of wall w of simulation s; the list holds global IDs
+  const int ch = blockIdx.x, w = blockIdx.y, sim = blockIdx.z;
   const int j0 = T.wrange[2 * sim + w], j1 = T.wrange[2 * sim + w + 1];
+  const int nch = max(1, (j1 - j0 + kWallChunk - 1) / kWallChunk);
+  if (ch >= nch) return;
+  const int c0 = j0 + ch * kWallChunk, c1 = min(j1, c0 + kWallChunk);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int j = j0 + threadIdx.x; j < j1; j += kRedBlock) {
+  for (int j = c0 + threadIdx.x; j < c1; j += kWallBlock) {
     int p = rank[B.wlist[j]];
     acc[0] += (double)D.f[0][p];
     acc[1] += (double)D.f[1][p];
     acc[2] += (double)D.f[2][p];
   }
-  block_sum<3, kRedBlock>(acc);
+  block_sum<3, kWallBlock>(acc);
+  const int q = 2 * sim + w;
+  __shared__ int last;
   if (threadIdx.x == 0) {
-    double *fw = B.fwall + 6 * sim;
-    fw[0 * 2 + w] = acc[0];
-    fw[1 * 2 + w] = acc[1];
-    fw[2 * 2 + w] = P.dim == 3 ? acc[2] : 0.0;
+    double *pp = part + ((size_t)q * maxch + ch) * 3;
+    pp[0] = acc[0];
+    pp[1] = acc[1];
+    pp[2] = acc[2];
+    __threadfence();  // the partial is visible before the count
+    last = atomicAdd(&cnt[q], 1u) == (unsigned)(nch - 1);
   }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();  // the other workgroups' partials are read after their counts
+  const double *pp = part + (size_t)q * maxch * 3;
+  double t[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nch; ++k)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t[a] += pp[3 * k + a];
+  cnt[q] = 0u;
+  double *fw = B.fwall + 6 * sim;
+  fw[0 * 2 + w] = t[0];
+  fw[1 * 2 + w] = t[1];
+  fw[2 * 2 + w] = T.P[sim].dim == 3 ? t[2] : 0.0;
 }
 
 template <class R>
@@ -2627,6 +2655,8 @@ class DeviceImpl final : public Device {
     HIPCHK(xc_.alloc(nclist_)); HIPCHK(yc_.alloc(nclist_)); HIPCHK(zc_.alloc(nclist_));
     HIPCHK(wlist_.alloc(nwlist_));
     HIPCHK(fwall_.alloc(6 * S));
+    HIPCHK(wcnt_.alloc(2 * S));
+    HIPCHK(hipMemset(wcnt_.p, 0, 2 * S * sizeof(unsigned)));
     HIPCHK(steps_.alloc(S)); HIPCHK(mag_.alloc(3 * S));
 
     // particle state, ID order = initial order; IDs are global (pbeg + local)
@@ -2744,6 +2774,11 @@ class DeviceImpl final : public Device {
     if ((rc = up(pbeg_, pbeg)) || (rc = up(cell_off_, coffc)) || (rc = up(coll_off_, coff)) ||
         (rc = up(coll_sim_, csim)) || (rc = up(wrange_, wr)))
       return rc;
+    // wall-force partial sums: chunks per wall (at least one)
+    wmaxch_ = 1;
+    for (int q = 0; q < 2 * S; ++q)
+      wmaxch_ = std::max(wmaxch_, (wr[q + 1] - wr[q] + kWallChunk - 1) / kWallChunk);
+    HIPCHK(wpart_.alloc((size_t)3 * 2 * S * wmaxch_));
     if (nclist_) {
       HIPCHK(hipMemcpy(clist_.p, clist.data(), nclist_ * sizeof(int), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(cowner_.p, cowner.data(), nclist_ * sizeof(int), hipMemcpyHostToDevice));
@@ -2935,7 +2970,8 @@ class DeviceImpl final : public Device {
   int force_walls(std::string &err) override {
     if (!any_walls_) return SPH_OK;
     Mark m(this, PH_BODIES);
-    k_walls<R><<<dim3(2, nsim()), kRedBlock, 0, stream_>>>(table(), derived(), bodies(), rank_.p);
+    k_walls<R><<<dim3(wmaxch_, 2, nsim()), kWallBlock, 0, stream_>>>(
+        table(), derived(), bodies(), rank_.p, wpart_.p, wcnt_.p, wmaxch_);
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
@@ -3524,6 +3560,9 @@ class DeviceImpl final : public Device {
   bool list_valid_ = false;
   DevBuf<double> ke_part_, ke_out_, obs_part_, obs_out_;
   DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_, mag_;
+  DevBuf<double> wpart_;   // wall-force partial sums, 3 per chunk (k_walls)
+  DevBuf<unsigned> wcnt_;  // [2 nsim] chunks of each wall summed so far (k_walls)
+  int wmaxch_ = 1;         // chunks of the longest wall list
   DevBuf<int> clist_, cstart_, cowner_, wlist_, steps_;
   DevBuf<sph_params_t> simP_;
   DevBuf<int> blk_sim_, blk_first_, pbeg_, cell_off_, coll_off_, coll_sim_, wrange_;
