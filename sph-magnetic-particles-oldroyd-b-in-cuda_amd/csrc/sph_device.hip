@@ -79,10 +79,8 @@ struct Particles {  // per-particle state that moves with the reorder
   R *c[6];  // xx xy yy xz yz zz
   R *mass;
   int *type, *id;
-  // (x, y, z, 0) rounded to float, in the current sorted order: the FP32 form
-  // tests candidates against the cut-off with one 16-byte load each.  Kept
-  // current wherever positions change (reorder, VV1, colloid placement, uploads).
-  float4 *p4;
+  // Packed copies of the positions, in the current sorted order, kept current
+  // wherever positions change (reorder, VV1, colloid placement, uploads).
   // FP64 form: (x, y, z, 0) in double, same order: one 32-byte access per
   // gathered neighbour instead of three scattered 8-byte ones (null in FP32).
   double4 *pd;
@@ -261,12 +259,11 @@ __device__ __forceinline__ float fix_r2(const float u[3], uint4 a, uint4 b) {
   return DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
 }
 
-// Writes the position copies of particle k: float4 (both forms), and double4
-// (FP64 form) or fixed point (FP32 form).
+// Writes the packed position copy of particle k: double4 (FP64 form) or fixed
+// point (FP32 form).
 template <class R>
 __device__ __forceinline__ void store_pos(const Particles<R> &S, const sph_params_t &P, int k,
                                           double x, double y, double z) {
-  S.p4[k] = make_float4((float)x, (float)y, (float)z, 0.0f);
   if constexpr (sizeof(R) == 4) S.pq[k] = fix_pos(fix_of(P), x, y, z);
   else S.pd[k] = make_double4(x, y, z, 0.0);
 }
@@ -1531,7 +1528,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     out[(size_t)(cnt >> 2) * N] = buf;
   }
   VL.count[i] = cnt;
-  V.ref[i] = S.p4[i];
+  // the displacement checks compare float positions with this reference
+  V.ref[i] = make_float4((float)xi, (float)yi, (float)zi, 0.0f);
   if (cnt > cap) atomicOr(&V.overflow[sim], 1);
 }
 
@@ -1732,7 +1730,7 @@ template <class R, int DIM, bool DO_GV, bool DO_F>
 __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particles<R> &S,
                                             const Derived<R> &D, int i, const PairSelf<R> &me,
                                             PairAcc<R> &acc, int store_gv, int write_dcdt,
-                                            int with_ext) {
+                                            int with_ext, int kick) {
   if (DO_F) {
     if (with_ext && me.t == 0) ext_force<R, DIM>(P, S.mass[i], S.y[i], acc.f);
     D.f[0][i] = acc.f[0];
@@ -1754,15 +1752,18 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
   }
   const R(&g)[3][3] = acc.g;
   const R tinv = R(1) / (R)P.tau;
-  const R cxx = S.c[0][i], cxy = S.c[1][i], cyy = S.c[2][i];
+  R c[6], dc[6];
+#pragma unroll
+  for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) c[k] = S.c[k][i];
+  const R cxx = c[0], cxy = c[1], cyy = c[2];
   if (DIM == 2) {
     R cgxx = cxx * g[0][0] + cxy * g[1][0], cgxy = cxx * g[0][1] + cxy * g[1][1];
     R cgyx = cxy * g[0][0] + cyy * g[1][0], cgyy = cxy * g[0][1] + cyy * g[1][1];
-    D.dcdt[0][i] = R(2) * cgxx + tinv * (R(1) - cxx);
-    D.dcdt[1][i] = cgxy + cgyx - tinv * cxy;
-    D.dcdt[2][i] = R(2) * cgyy + tinv * (R(1) - cyy);
+    dc[0] = R(2) * cgxx + tinv * (R(1) - cxx);
+    dc[1] = cgxy + cgyx - tinv * cxy;
+    dc[2] = R(2) * cgyy + tinv * (R(1) - cyy);
   } else {
-    const R cxz = S.c[3][i], cyz = S.c[4][i], czz = S.c[5][i];
+    const R cxz = c[3], cyz = c[4], czz = c[5];
     R cgxx = cxx * g[0][0] + cxy * g[1][0] + cxz * g[2][0];
     R cgxy = cxx * g[0][1] + cxy * g[1][1] + cxz * g[2][1];
     R cgxz = cxx * g[0][2] + cxy * g[1][2] + cxz * g[2][2];
@@ -1772,12 +1773,26 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
     R cgzx = cxz * g[0][0] + cyz * g[1][0] + czz * g[2][0];
     R cgzy = cxz * g[0][1] + cyz * g[1][1] + czz * g[2][1];
     R cgzz = cxz * g[0][2] + cyz * g[1][2] + czz * g[2][2];
-    D.dcdt[0][i] = R(2) * cgxx + tinv * (R(1) - cxx);
-    D.dcdt[1][i] = cgxy + cgyx - tinv * cxy;
-    D.dcdt[2][i] = R(2) * cgyy + tinv * (R(1) - cyy);
-    D.dcdt[3][i] = cgxz + cgzx - tinv * cxz;
-    D.dcdt[4][i] = cgyz + cgzy - tinv * cyz;
-    D.dcdt[5][i] = R(2) * cgzz + tinv * (R(1) - czz);
+    dc[0] = R(2) * cgxx + tinv * (R(1) - cxx);
+    dc[1] = cgxy + cgyx - tinv * cxy;
+    dc[2] = R(2) * cgyy + tinv * (R(1) - cyy);
+    dc[3] = cgxz + cgzx - tinv * cxz;
+    dc[4] = cgyz + cgzy - tinv * cyz;
+    dc[5] = R(2) * cgzz + tinv * (R(1) - czz);
+  }
+#pragma unroll
+  for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) D.dcdt[k][i] = dc[k];
+  if (DO_F && kick) {
+    // V_Verlet_step2 of this fluid particle (src/main.cu step order; the same
+    // arithmetic as k_vv2), from the force and dc/dt still in registers: the
+    // step needs no separate kick pass.  Only this thread touches particle i's
+    // v and c here; the neighbours' were read from the pair records.
+    const R hdt = (R)(0.5 * P.dt);
+    const R hdtm = hdt / S.mass[i];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) S.v[a][i] = me.v[a] + hdtm * acc.f[a];
+#pragma unroll
+    for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) S.c[k][i] = c[k] + hdt * dc[k];
   }
 }
 
@@ -1795,7 +1810,7 @@ template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_PAIRS_WAVES_F64) void k_pairs(
     SimTable T, Particles<R> S, Derived<R> D, Bodies Ball,
     const PairRec<R, DIM> *__restrict__ rec,
-    NList L, int store_gv, int write_dcdt, int with_ext, VList VL) {
+    NList L, int store_gv, int write_dcdt, int with_ext, int kick, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
   const int iend = T.pbeg[sim + 1];
   const int i0 = T.blk_first[b] + threadIdx.x;
@@ -1922,7 +1937,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     }
     if (!live) return;
     if (vok) {
-      pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+      pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
+                                     kick);
       return;
     }
   }
@@ -1952,7 +1968,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     L.queue[atomicAdd(L.qlen, 1)] = i;
     return;
   }
-  pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+  pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
+                                     kick);
 }
 
 // Particles whose neighbours did not all fit in the list: re-scan the strips
@@ -1964,7 +1981,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
                                                        const PairRec<R, DIM> *__restrict__ rec,
                                                        NList L,
                                                        const int *__restrict__ csg, int store_gv,
-                                                       int write_dcdt, int with_ext) {
+                                                       int write_dcdt, int with_ext, int kick) {
   const int n = *L.qlen;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const int i = L.queue[q];
@@ -2002,7 +2019,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
         }
       }
     });
-    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext);
+    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
+                                     kick);
   }
 }
 
@@ -2572,7 +2590,6 @@ class DeviceImpl final : public Device {
       for (int a = 0; a < 6; ++a) HIPCHK(c_[b][a].alloc(N_));
       HIPCHK(mass_[b].alloc(N_)); HIPCHK(type_[b].alloc(N_)); HIPCHK(id_[b].alloc(N_));
     }
-    HIPCHK(p4_.alloc(N_));
     if (sizeof(R) == 4) HIPCHK(pq_.alloc(N_));
     else HIPCHK(pd_.alloc(N_));
     HIPCHK(newpos_.alloc(N_)); HIPCHK(ptmp_.alloc(N_));
@@ -2921,7 +2938,8 @@ class DeviceImpl final : public Device {
   }
 
   template <bool GV, bool F>
-  int pairs(int store_gv, int write_dcdt, int with_ext, int phase, std::string &err) {
+  int pairs(int store_gv, int write_dcdt, int with_ext, int phase, std::string &err,
+            int kick = 0) {
     int rc = settle(err);
     if (rc) return rc;
     rc = ensure_list(err);
@@ -2935,12 +2953,13 @@ class DeviceImpl final : public Device {
       constexpr bool WALL = decltype(wallc)::value;
       k_pairs<R, DIM, WALL, GV, F><<<nblk_, block_, 0, stream_>>>(
           table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), store_gv,
-          write_dcdt, with_ext, vlist());
+          write_dcdt, with_ext, kick, vlist());
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
           table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), cs_.p, store_gv,
-          write_dcdt, with_ext);
+          write_dcdt, with_ext, kick);
     });
     HIPCHK(hipGetLastError());
+    if (kick) rec_valid_ = false;  // velocities and conformations moved
     return SPH_OK;
   }
 
@@ -2954,8 +2973,9 @@ class DeviceImpl final : public Device {
   }
 
   // gradient + dc/dt + forces + external force in one walk.
-  int pairs_all(bool store_gradv, std::string &err) {
-    return pairs<true, true>(store_gradv ? 1 : 0, 1, 1, PH_FORCES, err);
+  // `kick`: a step's fused pass also applies V_Verlet_step2 to fluid particles.
+  int pairs_all(bool store_gradv, std::string &err, bool kick = false) {
+    return pairs<true, true>(store_gradv ? 1 : 0, 1, 1, PH_FORCES, err, kick ? 1 : 0);
   }
 
   int external_forces(std::string &err) override {
@@ -3015,6 +3035,22 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
+  // calculate_forces and V_Verlet_step2 of a step: the fused pair pass applies
+  // the fluid kick as it writes each particle's force and dc/dt (the wall and
+  // colloid sums after it read forces and colloid particles only), so only the
+  // rigid bodies' half-kick is left.  Same arithmetic as forces_all + vv2.
+  int forces_and_kick(std::string &err) {
+    int rc;
+    if ((rc = density(err))) return rc;
+    if ((rc = pairs_all(false, err, true))) return rc;
+    if ((rc = force_walls(err))) return rc;
+    if ((rc = force_colloids(err))) return rc;
+    Mark m(this, PH_VV2);
+    if (ncoll_ > 0) k_coll_vv2<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+  }
+
   int vv2(std::string &err) override {
     Mark m(this, PH_VV2);
     rec_valid_ = false;
@@ -3047,8 +3083,7 @@ class DeviceImpl final : public Device {
     if ((rc = vv1(err))) return rc;
     if ((rc = reorder(nullptr, false, err))) return rc;
     if ((rc = set_step(step, err))) return rc;
-    if ((rc = forces_all(false, err))) return rc;
-    return vv2(err);
+    return forces_and_kick(err);
   }
 
   // The same step with Verlet lists: the displacement checks of VV1 decide, per
@@ -3069,8 +3104,7 @@ class DeviceImpl final : public Device {
       HIPCHK(hipGetLastError());
     }
     if ((rc = set_step(step, err))) return rc;
-    if ((rc = forces_all(false, err))) return rc;
-    return vv2(err);
+    return forces_and_kick(err);
   }
 
   int run_steps(int nsteps, int first_step, bool graph, std::string &err) override {
@@ -3438,7 +3472,6 @@ class DeviceImpl final : public Device {
     for (int a = 0; a < 3; ++a) S.v[a] = v_[b][a].p;
     for (int a = 0; a < 6; ++a) S.c[a] = c_[b][a].p;
     S.mass = mass_[b].p; S.type = type_[b].p; S.id = id_[b].p;
-    S.p4 = p4_.p;
     S.pd = pd_.p;
     S.pq = pq_.p;
     return S;
@@ -3543,7 +3576,6 @@ class DeviceImpl final : public Device {
   DevBuf<double> x_[2], y_[2], z_[2];
   DevBuf<R> v_[2][3], c_[2][6], mass_[2];
   DevBuf<int> type_[2], id_[2];
-  DevBuf<float4> p4_;
   DevBuf<double4> pd_;
   DevBuf<uint4> pq_;
   DevBuf<int> newpos_;  // old sorted position -> new, of the last stand-alone reorder

@@ -181,9 +181,10 @@ def test_bitwise_repeatable(precision, tmp_path):
 
 def test_method_sequence_matches_step(tmp_path):
     """VV1, neighbours_search, set_step, calculate_forces, VV2 called one by one
-    against sph_step, whose steps replay captured graphs (one per buffer parity)
-    and use Verlet lists (same pairs, list order instead of cell order):
-    FP64 agreement to 1e-12 after every replay, with host reads in between."""
+    against sph_step, whose steps replay a captured graph, use Verlet lists
+    (same pairs, list order instead of cell order) and apply VV2's fluid kick
+    inside the fused pair pass: FP64 agreement to 1e-12 after every replay, with
+    host reads in between."""
     a = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="a")
     b = helpers.make_system(helpers.cfg_colloids_2d(), str(tmp_path), name="b")
     a.prepare()
