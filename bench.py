@@ -46,8 +46,8 @@ def kernel_bytes(phase, dim, rbytes, store_gradv=False):
     (neighbour data re-reads and the neighbour list are not counted)."""
     pos = 8 * dim
     v, c = rbytes * dim, rbytes * (6 if dim == 3 else 3)
-    if phase == "forces":      # k_pairs: x v c type dens press mass -> F dc/dt
-        return pos + v + c + 4 + 3 * rbytes + v + c
+    if phase == "forces":      # k_pairs: x v c type dens press mass -> F dc/dt, kicked v c
+        return pos + v + c + 4 + 3 * rbytes + v + c + v + c
     if phase == "density":     # k_density: x mass -> dens press
         return pos + rbytes + 2 * rbytes
     if phase == "gradv_dcdt":  # x v type dens c -> dcdt (+ gradv)
@@ -63,6 +63,10 @@ def kernel_bytes(phase, dim, rbytes, store_gradv=False):
 
 
 TRAFFIC_FILE = "profiles/traffic_walls_colloids_1M_fp32.json"
+PAIR_PROFILE = "profiles/pair_passes_walls_colloids_1M_fp32.md"
+# VALU instructions x 4 cycles / SIMD busy time of the FP32 pair passes on the
+# 1M-particle workload (SQ_INSTS_VALU, SQ_BUSY_CYCLES; see PAIR_PROFILE)
+VALU_BUSY = {"k_pairs": 0.64, "k_density": 0.53}
 
 
 def measured_traffic(kernel, N):
@@ -308,6 +312,10 @@ def main():
             "traffic_source": TRAFFIC_FILE if traffic is not None else None,
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,
+            # what bounds it instead (SQ counters of the same kernel and workload,
+            # from the committed profile; not measured in this run)
+            "valu_busy_profiled": VALU_BUSY.get(kname),
+            "valu_busy_source": PAIR_PROFILE if kname in VALU_BUSY else None,
         },
         "sweep": sw,
         "sweep_pipeline": pipe,
