@@ -2103,7 +2103,10 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
   const bool d3 = P.dim == 3;
   const int j0 = Ball.cstart[g], j1 = Ball.cstart[g + 1];
   const double Xc = B.s[C_X][ci], Yc = B.s[C_Y][ci], Zc = B.s[C_Z][ci];
-  double acc[6] = {0, 0, 0, 0, 0, 0};  // fx fy fz tx ty tz
+  // fx fy fz tx ty tz over the colloid's particles, then the repulsion and
+  // magnetic forces from the other colloids (one colloid j per thread), all
+  // summed by one fixed-order block reduction
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int j = j0 + threadIdx.x; j < j1; j += kBlock) {
     int p = rank[B.clist[j]];
     double Fx = D.f[0][p], Fy = D.f[1][p], Fz = d3 ? (double)D.f[2][p] : 0.0;
@@ -2118,44 +2121,42 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
     acc[4] += rz * Fx - rx * Fz;
     acc[5] += rx * Fy - ry * Fx;
   }
-  block_sum<6, kBlock>(acc);
-  if (threadIdx.x != 0) return;
-  double F[3] = {acc[0], acc[1], acc[2]};
   if (ncoll >= 2) {
     const double mag[3] = {mag_all[3 * sim], mag_all[3 * sim + 1], mag_all[3 * sim + 2]};
-    // Repulsion, then magnetic force, over all other colloids within their
+    // Repulsion and magnetic force from every other colloid within their
     // cut-offs (the set the reference finds through its colloid cell list).
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int j = 0; j < ncoll; ++j) {
-        if (j == ci) continue;
-        double r[3];
-        r[0] = mimg(Xc - B.s[C_X][j], P.L[0], 0.5 * P.L[0]);
-        r[1] = Yc - B.s[C_Y][j];
-        if (P.wall == 0) r[1] = mimg(r[1], P.L[1], 0.5 * P.L[1]);
-        r[2] = d3 ? mimg(Zc - B.s[C_Z][j], P.L[2], 0.5 * P.L[2]) : 0.0;
-        double r2 = d3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-        if (pass == 0) {
-          if (r2 < P.rcuton_coll_rep_sq) r2 = P.rcuton_coll_rep_sq;
-          if (r2 <= P.rcutoff_coll_rep_sq) {
-            double rr = sqrt(r2);
-            double ts = -P.tau_rep * (rr - 2.0 * P.coll_R) / P.coll_R;
-            double ex = exp(ts);
-            double Fm = (P.F0_rep * P.tau_rep) * ex / (1.0 - ex);
-            for (int a = 0; a < P.dim; ++a) F[a] = F[a] + Fm * (r[a] / rr);
-          }
-        } else if (r2 <= P.r0_magnet_sq) {
-          double rr = d3 ? sqrt(r2) / P.coll_R : sqrt(r2);
-          double e[3] = {r[0] / rr, r[1] / rr, d3 ? r[2] / rr : 0.0};
-          double me = d3 ? mag[0] * e[0] + mag[1] * e[1] + mag[2] * e[2]
-                         : mag[0] * e[0] + mag[1] * e[1];
-          double me2 = me * me;
-          double F0r4 = P.F0_magnet / (r2 * r2);
-          for (int a = 0; a < P.dim; ++a)
-            F[a] = F[a] + F0r4 * (2.0 * me * mag[a] - (5.0 * me2 - 1.0) * e[a]);
-        }
+    for (int j = threadIdx.x; j < ncoll; j += kBlock) {
+      if (j == ci) continue;
+      double r[3];
+      r[0] = mimg(Xc - B.s[C_X][j], P.L[0], 0.5 * P.L[0]);
+      r[1] = Yc - B.s[C_Y][j];
+      if (P.wall == 0) r[1] = mimg(r[1], P.L[1], 0.5 * P.L[1]);
+      r[2] = d3 ? mimg(Zc - B.s[C_Z][j], P.L[2], 0.5 * P.L[2]) : 0.0;
+      const double r2 = d3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+      double rs2 = r2;
+      if (rs2 < P.rcuton_coll_rep_sq) rs2 = P.rcuton_coll_rep_sq;
+      if (rs2 <= P.rcutoff_coll_rep_sq) {
+        double rr = sqrt(rs2);
+        double ts = -P.tau_rep * (rr - 2.0 * P.coll_R) / P.coll_R;
+        double ex = exp(ts);
+        double Fm = (P.F0_rep * P.tau_rep) * ex / (1.0 - ex);
+        for (int a = 0; a < P.dim; ++a) acc[6 + a] += Fm * (r[a] / rr);
+      }
+      if (r2 <= P.r0_magnet_sq) {
+        double rr = d3 ? sqrt(r2) / P.coll_R : sqrt(r2);
+        double e[3] = {r[0] / rr, r[1] / rr, d3 ? r[2] / rr : 0.0};
+        double me = d3 ? mag[0] * e[0] + mag[1] * e[1] + mag[2] * e[2]
+                       : mag[0] * e[0] + mag[1] * e[1];
+        double me2 = me * me;
+        double F0r4 = P.F0_magnet / (r2 * r2);
+        for (int a = 0; a < P.dim; ++a)
+          acc[6 + a] += F0r4 * (2.0 * me * mag[a] - (5.0 * me2 - 1.0) * e[a]);
       }
     }
   }
+  block_sum<9, kBlock>(acc);
+  if (threadIdx.x != 0) return;
+  double F[3] = {acc[0] + acc[6], acc[1] + acc[7], acc[2] + acc[8]};
   B.s[C_FX][ci] = F[0];
   B.s[C_FY][ci] = F[1];
   B.s[C_FZ][ci] = d3 ? F[2] : 0.0;
