@@ -872,18 +872,40 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *t
                                                      const int *any) {
   if (any && !*any) return;
   const int c0 = blockIdx.x * kScanTile + threadIdx.x * 8;
+  // a thread's 8 cells as two 16-byte accesses (c0 is a multiple of 8 ints)
+  // where all 8 exist, element by element at the end of the array
+  const bool whole = c0 + 8 <= ncells;
   int v[8];
+  if (whole) {
+    const int4 a = *reinterpret_cast<const int4 *>(count + c0);
+    const int4 b = *reinterpret_cast<const int4 *>(count + c0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = c0 + k < ncells ? count[c0 + k] : 0;
+    for (int k = 0; k < 8; ++k) v[k] = c0 + k < ncells ? count[c0 + k] : 0;
+  }
   int total;
   int run = block_scan8(v, &total);
+  int o[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (c0 + k < ncells) {
-      cs[c0 + k] = run;
-      count[c0 + k] = 0;
-    }
+    o[k] = run;
     run += v[k];
+  }
+  if (whole) {
+    *reinterpret_cast<int4 *>(cs + c0) = make_int4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<int4 *>(cs + c0 + 4) = make_int4(o[4], o[5], o[6], o[7]);
+    *reinterpret_cast<int4 *>(count + c0) = make_int4(0, 0, 0, 0);
+    *reinterpret_cast<int4 *>(count + c0 + 4) = make_int4(0, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (c0 + k < ncells) {
+        cs[c0 + k] = o[k];
+        count[c0 + k] = 0;
+      }
+    }
   }
   if (threadIdx.x == 0) tsum[blockIdx.x] = total;
 }
