@@ -646,12 +646,26 @@ struct PairSelf {
   }
 };
 
+// The colloid state the Morris velocity of a wall/colloid pair reads (fields
+// C_X .. C_OZ: position, velocity, angular velocity of colloid c of the pair's
+// simulation): from the Bodies arrays, or from a copy the pair pass staged in
+// LDS, whose reads do not wait for the record loads in flight.
+struct CollFromBodies {
+  const Bodies &B;
+  __device__ __forceinline__ double operator()(int f, int c) const { return B.s[f][c]; }
+};
+constexpr int kCollStage = 32;  // colloids per simulation the pair pass stages in LDS
+struct CollFromLds {
+  const double (&t)[C_OZ + 1][kCollStage];
+  __device__ __forceinline__ double operator()(int f, int c) const { return t[f][c]; }
+};
+
 // Relative velocity v_ij with Morris boundary conditions for wall and colloid
 // partners (src/kernel_vel_gradients.cu:157-332, 415-630; the same branches in
 // src/kernel_forces.cu:196-419, 533-822).  Called only when ti == tj == 0 or
 // ti != tj.
-template <class R, int DIM, bool WALL>
-__device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const Bodies &B, int ti,
+template <class R, int DIM, bool WALL, class CV>
+__device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, int ti,
                                            int tj, double xi, double yi, double zi,
                                            double xj, double yj, double zj, const R vi[3],
                                            const R vj[3], const R rij[3], R v[3]) {
@@ -679,20 +693,20 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const Bodies 
     } else {
       const int c = tj - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B.s[C_X][c], B.s[C_Y][c],
-                             DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
+      pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B(C_X, c), B(C_Y, c),
+                             DIM == 3 ? B(C_Z, c) : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R dis = dc - Rc;
-      const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
+      const R Vc[3] = {(R)B(C_VX, c), (R)B(C_VY, c), DIM == 3 ? (R)B(C_VZ, c) : R(0)};
       if (dis < R(0)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = vi[a] - Vc[a];
       } else {
         R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
         R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
-        R ox = DIM == 3 ? (R)B.s[C_OX][c] : R(0), oy = DIM == 3 ? (R)B.s[C_OY][c] : R(0);
-        R oz = (R)B.s[C_OZ][c];
+        R ox = DIM == 3 ? (R)B(C_OX, c) : R(0), oy = DIM == 3 ? (R)B(C_OY, c) : R(0);
+        R oz = (R)B(C_OZ, c);
         R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
         R djs = (rij[0] * sv[0] + rij[1] * sv[1] + rij[2] * sv[2]) - dis;
         R beta = R(1) + djs / dis;
@@ -719,20 +733,20 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const Bodies 
     } else {
       const int c = ti - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B.s[C_X][c], B.s[C_Y][c],
-                             DIM == 3 ? B.s[C_Z][c] : 0.0, rc);
+      pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B(C_X, c), B(C_Y, c),
+                             DIM == 3 ? B(C_Z, c) : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R djs = dc - Rc;
-      const R Vc[3] = {(R)B.s[C_VX][c], (R)B.s[C_VY][c], DIM == 3 ? (R)B.s[C_VZ][c] : R(0)};
+      const R Vc[3] = {(R)B(C_VX, c), (R)B(C_VY, c), DIM == 3 ? (R)B(C_VZ, c) : R(0)};
       if (djs < R(0)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = Vc[a] - vj[a];
       } else {
         R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
         R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
-        R ox = DIM == 3 ? (R)B.s[C_OX][c] : R(0), oy = DIM == 3 ? (R)B.s[C_OY][c] : R(0);
-        R oz = (R)B.s[C_OZ][c];
+        R ox = DIM == 3 ? (R)B(C_OX, c) : R(0), oy = DIM == 3 ? (R)B(C_OY, c) : R(0);
+        R oz = (R)B(C_OZ, c);
         R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
         R dis = (-rij[0] * sv[0] - rij[1] * sv[1] - rij[2] * sv[2]) - djs;
         R beta = R(1) + dis / djs;
@@ -1629,9 +1643,9 @@ struct PairAcc {
 // One neighbour j of i: velocity-gradient and force contributions with the
 // reference's pair selection, Morris velocities and Pi/d^2 substitutions
 // (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
-template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
+template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F, class CV>
 __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<R> &rj,
-                                          const Bodies &B, const PairSelf<R> &me,
+                                          const CV &B, const PairSelf<R> &me,
                                           PairAcc<R> &acc) {
   const int tj = rj.w < R(0) ? (int)(-rj.w) : 0;
   const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
@@ -1828,7 +1842,9 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // here from one evaluation of r, e, gradW and the Morris relative velocity.
 // DO_GV / DO_F select the parts.  A particle with more neighbours than the list
 // holds parks its partial sums in gv/f and joins the queue for k_pairs_tail.
-template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F>
+// CST: the simulations' colloid states are staged in LDS (every simulation of the
+// batch has at most kCollStage colloids).
+template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F, bool CST>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_PAIRS_WAVES_F64) void k_pairs(
     SimTable T, Particles<R> S, Derived<R> D, Bodies Ball,
     const PairRec<R, DIM> *__restrict__ rec,
@@ -1837,13 +1853,29 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   const int iend = T.pbeg[sim + 1];
   const int i0 = T.blk_first[b] + threadIdx.x;
   const bool live = i0 < iend;
+  const sph_params_t &P = T.P[sim];
+  const Bodies B = bodies_of(Ball, T, sim);
+  // the colloid states of the block's simulation, staged before any thread leaves
+  __shared__ double cst[C_OZ + 1][kCollStage];
+  if constexpr (CST) {
+    const int nc = P.N_colloids;  // (uniform: one simulation per workgroup)
+    if (nc > 0) {
+      // (fields unrolled: a run-time field index would put B.s in scratch)
+      for (int c = threadIdx.x; c < nc; c += blockDim.x)
+#pragma unroll
+        for (int f = 0; f <= C_OZ; ++f) cst[f][c] = B.s[f][c];
+      __syncthreads();
+    }
+  }
+  const auto cv = [&] {
+    if constexpr (CST) return CollFromLds{cst};
+    else return CollFromBodies{B};
+  }();
   // Verlet walk: every lane of a wave takes part in the quad loads, so the
   // lanes past the simulation's end stay (aliasing its last particle for their
   // reads) and store nothing.
   if (!live && !VL.idx) return;
   const int i = live ? i0 : iend - 1;
-  const sph_params_t &P = T.P[sim];
-  const Bodies B = bodies_of(Ball, T, sim);
   const int N = T.ntotal;  // list stride
   PairSelf<R> me;
   load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
@@ -1919,23 +1951,61 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
         v[k] = __builtin_amdgcn_raw_buffer_load_b128(rb.rs, o + 16 * chk(k), 0, 0);
       }
     };
-    // entry e: evaluate the chunks in cur, quad-load entry e + 1 into nxt
-    auto entry = [&](int e, const u4(&cur)[kLoads], u4(&nxt)[kLoads]) {
-      const bool hit = m & 1u;
+    // The loader (q, m) runs kAhead entries ahead of the evaluation: its record
+    // loads are in flight for kAhead entries of work.  One entry ahead where a
+    // deeper pipeline would cost occupancy; two for the two-chunk record (2D
+    // FP32), whose small batches leave the GPU under-filled and each wavefront
+    // waiting on load latency.
+    constexpr int kAhead = kChunks <= 2 ? 2 : 1;
+    // Entries per trip, and chunk buffers (each entry's buffer is fixed within
+    // a trip; with two ahead only three of the four are live at a time).  Four
+    // entries per trip start every trip at a multiple of four, so that each
+    // entry's place in its group of four indices is known at compile time.
+    constexpr int kTrip = kAhead == 1 ? 2 : 4;
+    // loader: from entry k - 1 to entry k (bit 0 of m then describes k; its
+    // index is q.x of the rotating q with one entry ahead, else component
+    // k & 3 of the group q, a compile-time choice)
+    auto advance = [&](int k) {
       m >>= 1;
-      q = make_int4(q.y, q.z, q.w, q.x);
-      if (((e + 1) & 3) == 0) {
+      if constexpr (kAhead == 1) q = make_int4(q.y, q.z, q.w, q.x);
+      if ((k & 3) == 0) {
         q = qn;
-        qn = vl[(size_t)min((e + 5) >> 2, lastg) * N];
+        qn = vl[(size_t)min((k + 4) >> 2, lastg) * N];
       }
-      if (((e + 1) & 31) == 0) {
+      if ((k & 31) == 0) {
         m = mn;
         mn = mn2;
-        mn2 = vm[(size_t)min((e + 65) >> 5, lastw) * N];
-        const int left = vc - (e + 1);
+        mn2 = vm[(size_t)min((k + 64) >> 5, lastw) * N];
+        const int left = vc - k;
         if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
       }
-      quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(Rec) : oob, nxt);
+    };
+    auto load_at = [&](int k, u4(&v)[kLoads]) {
+      int j = q.x;
+      if constexpr (kAhead > 1) {
+        const int c = k & 3;
+        j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
+      }
+      quad_load((m & 1u) ? (unsigned)(j - pb) * (unsigned)sizeof(Rec) : oob, v);
+    };
+    // pend: hit bits of the entries loaded but not yet evaluated (bit 0: the
+    // next to evaluate)
+    u4 buf[kTrip][kLoads];
+    unsigned pend = m & 1u;
+    load_at(0, buf[0]);
+#pragma unroll
+    for (int j = 1; j < kAhead; ++j) {
+      advance(j);
+      pend |= (m & 1u) << j;
+      load_at(j, buf[j]);
+    }
+    // entry e: load entry e + kAhead into nxt, evaluate the chunks in cur
+    auto entry = [&](int e, const u4(&cur)[kLoads], u4(&nxt)[kLoads]) {
+      advance(e + kAhead);
+      pend |= (m & 1u) << kAhead;
+      const bool hit = pend & 1u;
+      pend >>= 1;
+      load_at(e + kAhead, nxt);
 #pragma unroll
       for (int k = 0; k < kLoads; ++k) st[qa + src(k)][chk(k)] = cur[k];
       __builtin_amdgcn_wave_barrier();
@@ -1943,19 +2013,17 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
 #pragma unroll
       for (int k = 0; k < kRecVecs<R, DIM>; ++k) mine[k] = st[lane][k];
       __builtin_amdgcn_wave_barrier();
-      if (hit) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, unpack_chunks<R, DIM>(mine), B, me, acc);
+      if (hit) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, unpack_chunks<R, DIM>(mine), cv, me, acc);
     };
-    // two entries per trip with the chunk buffers swapping roles (no copies)
-    u4 ca[kLoads], cb[kLoads];
-    quad_load((m & 1u) ? (unsigned)(q.x - pb) * (unsigned)sizeof(Rec) : oob, ca);
-    // (Always both entries of a trip: with a trip that could end after the first,
-    // the loads of the first would still be pending at the loop head and every
-    // trip would wait for them before issuing its own.  Entry vmax, evaluated
-    // when vmax is odd, is past every lane's list: no hit, no record access.)
+    // kTrip entries per trip with the chunk buffers rotating roles (no copies).
+    // (Always all entries of a trip: with a trip that could end early, the
+    // loads of its last entries would still be pending at the loop head and
+    // every trip would wait for them before issuing its own.  Entries past
+    // vmax are past every lane's list: no hit, no record access.)
 #pragma unroll 1
-    for (int e = 0; e < vmax; e += 2) {
-      entry(e, ca, cb);
-      entry(e + 1, cb, ca);
+    for (int e = 0; e < vmax; e += kTrip) {
+#pragma unroll
+      for (int t = 0; t < kTrip; ++t) entry(e + t, buf[t], buf[(t + kAhead) % kTrip]);
     }
     if (!live) return;
     if (vok) {
@@ -1975,7 +2043,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const int m = min(4, cnt - k);
 #pragma unroll 1
     for (int u = 0; u < m; ++u) {
-      pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R, DIM>(rbn, q.x - pbn), B, me, acc);
+      pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec<R, DIM>(rbn, q.x - pbn), cv, me, acc);
       q = make_int4(q.y, q.z, q.w, 0);
     }
   }
@@ -2036,7 +2104,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
         c.load(S, k);
         const R r2 = (R)cme.r2(c);
         if (r2 < rcutsq && k != i) {
-          if (hit >= skip) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec(rec, k), B, me, acc);
+          if (hit >= skip)
+            pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec(rec, k), CollFromBodies{B}, me, acc);
           hit++;
         }
       }
@@ -2572,6 +2641,7 @@ class DeviceImpl final : public Device {
     sims_.clear();
     N_ = ncells_ = ncoll_ = nclist_ = nwlist_ = 0;
     any_walls_ = false;
+    coll_lds_ = true;
     for (const HostState *h : hs) {
       const sph_params_t &p = h->p;
       if (p.dim != P_.dim || p.wall != P_.wall) {
@@ -2600,6 +2670,7 @@ class DeviceImpl final : public Device {
       nclist_ += p.Nlist_colloids;
       nwlist_ += p.wall ? p.Nlist_walls : 0;
       any_walls_ = any_walls_ || (p.wall && p.Nwalls > 0);
+      coll_lds_ = coll_lds_ && p.N_colloids <= kCollStage;
     }
     const int S = nsim();
     nc_ = P_.dim == 3 ? 6 : 3;
@@ -2974,9 +3045,14 @@ class DeviceImpl final : public Device {
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_pairs<R, DIM, WALL, GV, F><<<nblk_, block_, 0, stream_>>>(
-          table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), store_gv,
-          write_dcdt, with_ext, kick, vlist());
+      if (coll_lds_)
+        k_pairs<R, DIM, WALL, GV, F, true><<<nblk_, block_, 0, stream_>>>(
+            table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), store_gv,
+            write_dcdt, with_ext, kick, vlist());
+      else
+        k_pairs<R, DIM, WALL, GV, F, false><<<nblk_, block_, 0, stream_>>>(
+            table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), store_gv,
+            write_dcdt, with_ext, kick, vlist());
       k_pairs_tail<R, DIM, WALL, GV, F><<<kTailBlocks, kBlock, 0, stream_>>>(
           table(), parts(cur_), derived(), bodies(), recs<DIM>(), nlist(), cs_.p, store_gv,
           write_dcdt, with_ext, kick);
@@ -3577,6 +3653,7 @@ class DeviceImpl final : public Device {
   std::vector<SimInfo> sims_;
   int N_ = 0, ncells_ = 0, ncoll_ = 0, nclist_ = 0, nwlist_ = 0;
   bool any_walls_ = false;
+  bool coll_lds_ = true;  // every simulation's colloids fit the pair pass's LDS stage
   int nc_ = 3, cur_ = 0;
   int block_ = kBlock, xcd_ = 1, nblk_ = 0, max_parts_ = 1;
   // Verlet lists (used by run_steps): enabled when skin > 0 and every periodic
