@@ -1233,8 +1233,18 @@ __device__ __forceinline__ void pressure_terms(const sph_params_t &P, const Part
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable T, Particles<R> S, Derived<R> D,
                                                     PairRec<R, DIM> *rec, const int *__restrict__ csg,
-                                                    NList L, int write_dens, VList VL) {
+                                                    NList L, int write_dens, VList VL,
+                                                    VerletCtl V, int verlet_done) {
   const int b = tile_of(T), sim = T.blk_sim[b];
+  if (verlet_done && threadIdx.x == 0 && T.blk_first[b] == T.pbeg[sim]) {
+    // After a Verlet step's list build (the previous launch): a simulation
+    // whose list overflowed is rebuilt again next step (no thread of this pass
+    // reads the flags).
+    const int of = V.overflow[sim];
+    V.rebuild[sim] = of ? 1 : 0;
+    if (of) atomicOr(V.any, 1);  // (the next step's flag)
+    V.overflow[sim] = 0;
+  }
   const sph_params_t &P = T.P[sim];
   const int *__restrict__ cs = csg + T.cell_off[sim];
   const int N = T.ntotal;  // list stride
@@ -1569,14 +1579,6 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   if (cnt > cap) atomicOr(&V.overflow[sim], 1);
 }
 
-// After a build: simulations whose lists overflowed are rebuilt again next step.
-__global__ void k_verlet_done(VerletCtl V, int nsim) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsim) return;
-  V.rebuild[s] = V.overflow[s] ? 1 : 0;
-  if (V.overflow[s]) atomicOr(V.any, 1);  // (the next step's flag)
-  V.overflow[s] = 0;
-}
 
 __global__ void k_fill(int *a, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2185,7 +2187,8 @@ __global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S, Derived<R> D,
                                                      Bodies Ball, const int *__restrict__ rank,
-                                                     const double *__restrict__ mag_all) {
+                                                     const double *__restrict__ mag_all,
+                                                     int kick) {
   // one block per colloid of the batch; ci is its index in its simulation
   const int g = blockIdx.x, sim = T.coll_sim[g];
   const sph_params_t &P = T.P[sim];
@@ -2254,6 +2257,19 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
   B.s[C_TX][ci] = d3 ? acc[3] : 0.0;
   B.s[C_TY][ci] = d3 ? acc[4] : 0.0;
   B.s[C_TZ][ci] = acc[5];
+  if (kick && P.coll_move == 0) {
+    // the colloid's V_Verlet_step2 half-kick (k_coll_vv2's arithmetic) from the
+    // force and torque just stored; the other blocks read colloid positions only
+    const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
+    B.s[C_VX][ci] += hdtm * F[0];
+    B.s[C_VY][ci] += hdtm * F[1];
+    if (d3) B.s[C_VZ][ci] += hdtm * F[2];
+    B.s[C_OZ][ci] += hdtI * acc[5];
+    if (d3) {
+      B.s[C_OX][ci] += hdtI * acc[3];
+      B.s[C_OY][ci] += hdtI * acc[4];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3005,9 +3021,9 @@ class DeviceImpl final : public Device {
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_density<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(),
-                                                              recs<DIM>(), cs_.p, nlist(),
-                                                              write_dens, vlist());
+      k_density<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(
+          table(), parts(cur_), derived(), recs<DIM>(), cs_.p, nlist(), write_dens, vlist(),
+          vctl(), in_verlet_ ? 1 : 0);
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
@@ -3095,11 +3111,14 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
-  int force_colloids(std::string &err) override {
+  int force_colloids(std::string &err) override { return colloid_forces(false, err); }
+
+  // `kick`: a step's colloid pass also applies the colloids' V_Verlet_step2.
+  int colloid_forces(bool kick, std::string &err) {
     if (ncoll_ == 0) return SPH_OK;
     Mark m(this, PH_BODIES);
     k_colloids<R><<<ncoll_, kBlock, 0, stream_>>>(table(), parts(cur_), derived(), bodies(),
-                                                  rank_.p, mag_.p);
+                                                  rank_.p, mag_.p, kick ? 1 : 0);
     HIPCHK(hipGetLastError());
     return SPH_OK;
   }
@@ -3136,18 +3155,15 @@ class DeviceImpl final : public Device {
 
   // calculate_forces and V_Verlet_step2 of a step: the fused pair pass applies
   // the fluid kick as it writes each particle's force and dc/dt (the wall and
-  // colloid sums after it read forces and colloid particles only), so only the
-  // rigid bodies' half-kick is left.  Same arithmetic as forces_all + vv2.
+  // colloid sums after it read forces and colloid particles only), and the
+  // colloid pass the colloids' kick as it writes their force and torque.  Same
+  // arithmetic as forces_all + vv2.
   int forces_and_kick(std::string &err) {
     int rc;
     if ((rc = density(err))) return rc;
     if ((rc = pairs_all(false, err, true))) return rc;
     if ((rc = force_walls(err))) return rc;
-    if ((rc = force_colloids(err))) return rc;
-    Mark m(this, PH_VV2);
-    if (ncoll_ > 0) k_coll_vv2<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
-    HIPCHK(hipGetLastError());
-    return SPH_OK;
+    return colloid_forces(true, err);
   }
 
   int vv2(std::string &err) override {
@@ -3199,7 +3215,7 @@ class DeviceImpl final : public Device {
         k_verlet_build<DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), cs_.p,
                                                                  vlist(), vctl());
       });
-      k_verlet_done<<<blocks(nsim()), kBlock, 0, stream_>>>(vctl(), nsim());
+      // (the density pass that follows turns overflows into next step's rebuilds)
       HIPCHK(hipGetLastError());
     }
     if ((rc = set_step(step, err))) return rc;
