@@ -2688,6 +2688,8 @@ class DeviceImpl final : public Device {
       any_walls_ = any_walls_ || (p.wall && p.Nwalls > 0);
       coll_lds_ = coll_lds_ && p.N_colloids <= kCollStage;
     }
+    // SPH_COLLOID_LDS=0 keeps the colloid states in global memory (same results)
+    if (const char *e = std::getenv("SPH_COLLOID_LDS")) coll_lds_ = coll_lds_ && std::atoi(e) != 0;
     const int S = nsim();
     nc_ = P_.dim == 3 ? 6 : 3;
     max_parts_ = 1;

@@ -348,6 +348,28 @@ def test_small_neighbour_list_capacity_gives_identical_results(tmp_path, monkeyp
         assert relerr(a, b) <= 1e-12, n
 
 
+@pytest.mark.parametrize("precision", [sph_system.FP64, sph_system.FP32])
+def test_colloid_states_from_lds_or_global_memory(precision, tmp_path, monkeypatch):
+    """The pair pass reads the colloid states that the Morris velocities of
+    fluid-colloid pairs need from an LDS copy (at most 32 colloids per
+    simulation) or from global memory (more colloids; SPH_COLLOID_LDS=0 forces
+    it): the same values either way, so the trajectories are bitwise equal."""
+    outs = []
+    names = ("x", "vx", "vy", "fx", "dcdt_xy", "fx_colloid", "tz_colloid")
+    for lds in ("1", "0"):
+        monkeypatch.setenv("SPH_COLLOID_LDS", lds)
+        s = helpers.make_system(helpers.cfg_colloids_3d(n=24), str(tmp_path), precision,
+                                name=f"l{lds}")
+        helpers.perturb(s)
+        s.prepare()
+        s.step(3)
+        s.synchronize()
+        outs.append([s.get(n) for n in names])
+        s.close()
+    for n, a, b in zip(names, *outs):
+        np.testing.assert_array_equal(a, b, err_msg=n)
+
+
 @pytest.mark.parametrize("case", ["colloids3d", "colloids2d", "sine3d"])
 def test_observables(case, tmp_path):
     s, o = _pair(case, str(tmp_path))
