@@ -158,7 +158,9 @@ int sph_set_step(sph_handle h, int step);
 int sph_prepare(sph_handle h);
 /* `nsteps` iterations of the loop body of src/main.cu:358-459 without output:
  * VV1, neighbours_search, magnetic moment, calculate_forces, VV2.  Starts at
- * the handle's step counter + 1 and advances it.  Asynchronous. */
+ * the handle's step counter + 1 and advances it.  Asynchronous.  (Inside a
+ * step the VV2 kicks are applied by the force and colloid passes as they write
+ * each force; the arithmetic is that of the method sequence.) */
 int sph_step(sph_handle h, int nsteps);
 int sph_synchronize(sph_handle h);
 int sph_current_step(sph_handle h);
@@ -194,8 +196,9 @@ int sph_write_restart(sph_handle h, const char *path);
 /* ---- instrumentation ----------------------------------------------------- */
 /* When enabled, each device phase is bracketed by HIP events on the handle's
  * stream.  Phases: 0 VV1, 1 neighbour search, 2 density, 3 velocity gradient
- * + dc/dt, 4 forces, 5 walls + colloids, 6 VV2.  `ms` receives the summed
- * milliseconds per phase since the last reset and `calls` the launch counts. */
+ * + dc/dt, 4 forces, 5 walls + colloids, 6 VV2 (in sph_step, VV2 is part of
+ * phases 4 and 5).  `ms` receives the summed milliseconds per phase since the
+ * last reset and `calls` the launch counts. */
 int sph_set_timing(sph_handle h, int enable);
 int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]);
 
