@@ -2160,6 +2160,17 @@ __global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, 
   }
   block_sum<3, kWallBlock>(acc);
   const int q = 2 * sim + w;
+  if (nch == 1) {
+    // a wall of one chunk (every 2D wall of a sweep): its workgroup is the only
+    // one, so no counter and no device-scope fence (which writes back the L2 of
+    // its XCD; thousands of them cost a batch more than the sums)
+    if (threadIdx.x != 0) return;
+    double *fw = B.fwall + 6 * sim;
+    fw[0 * 2 + w] = 0.0 + acc[0];
+    fw[1 * 2 + w] = 0.0 + acc[1];
+    fw[2 * 2 + w] = T.P[sim].dim == 3 ? 0.0 + acc[2] : 0.0;
+    return;
+  }
   __shared__ int last;
   if (threadIdx.x == 0) {
     double *pp = part + ((size_t)q * maxch + ch) * 3;
