@@ -1925,9 +1925,9 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const unsigned oob =
         (unsigned)(__builtin_amdgcn_readfirstlane(iend) - pb) * (unsigned)sizeof(Rec);
     const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
-    // q: indices of the current group of four, rotated so that q.x is the
-    // next entry; qn: the following group.  m: hit bits of the current 32
-    // entries, bit 0 the next entry (masked to zero past the list); mn, mn2: the
+    // q: indices of the loader's current group of four; qn: the following
+    // group.  m: hit bits of the current 32 entries, bit 0 the loader's entry
+    // (masked to zero past the list); mn, mn2: the
     // following two words (unmasked until used; two ahead so that the word
     // taken over is never one still being loaded).
     int4 q = vl[0];
@@ -1960,16 +1960,16 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     // waiting on load latency.
     constexpr int kAhead = kChunks <= 2 ? 2 : 1;
     // Entries per trip, and chunk buffers (each entry's buffer is fixed within
-    // a trip; with two ahead only three of the four are live at a time).  Four
-    // entries per trip start every trip at a multiple of four, so that each
-    // entry's place in its group of four indices is known at compile time.
-    constexpr int kTrip = kAhead == 1 ? 2 : 4;
+    // a trip; only kAhead + 1 of the four are live at a time).  Four entries
+    // per trip start every trip at a multiple of four, so that each entry's
+    // place in its group of four indices is known at compile time (a rotating
+    // group would cost moves per entry, and with three entries per trip copies
+    // onto registers still being loaded).
+    constexpr int kTrip = 4;
     // loader: from entry k - 1 to entry k (bit 0 of m then describes k; its
-    // index is q.x of the rotating q with one entry ahead, else component
-    // k & 3 of the group q, a compile-time choice)
+    // index is component k & 3 of the group q)
     auto advance = [&](int k) {
       m >>= 1;
-      if constexpr (kAhead == 1) q = make_int4(q.y, q.z, q.w, q.x);
       if ((k & 3) == 0) {
         q = qn;
         qn = vl[(size_t)min((k + 4) >> 2, lastg) * N];
@@ -1983,11 +1983,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
       }
     };
     auto load_at = [&](int k, u4(&v)[kLoads]) {
-      int j = q.x;
-      if constexpr (kAhead > 1) {
-        const int c = k & 3;
-        j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
-      }
+      const int c = k & 3;  // (known at compile time, see kTrip)
+      const int j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
       quad_load((m & 1u) ? (unsigned)(j - pb) * (unsigned)sizeof(Rec) : oob, v);
     };
     // pend: hit bits of the entries loaded but not yet evaluated (bit 0: the
