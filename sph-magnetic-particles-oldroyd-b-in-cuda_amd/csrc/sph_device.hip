@@ -650,14 +650,22 @@ struct PairSelf {
 // C_X .. C_OZ: position, velocity, angular velocity of colloid c of the pair's
 // simulation): from the Bodies arrays, or from a copy the pair pass staged in
 // LDS, whose reads do not wait for the record loads in flight.
+// fixc(c): the centre in the fixed-point coordinates of the simulation's
+// positions (for the forms whose pair records hold fixed-point positions).
 struct CollFromBodies {
   const Bodies &B;
+  const sph_params_t &P;
   __device__ __forceinline__ double operator()(int f, int c) const { return B.s[f][c]; }
+  __device__ __forceinline__ uint4 fixc(int c) const {
+    return fix_pos(fix_of(P), B.s[C_X][c], B.s[C_Y][c], B.s[C_Z][c]);
+  }
 };
 constexpr int kCollStage = 32;  // colloids per simulation the pair pass stages in LDS
 struct CollFromLds {
   const double (&t)[C_OZ + 1][kCollStage];
+  const uint4 (&q)[kCollStage];
   __device__ __forceinline__ double operator()(int f, int c) const { return t[f][c]; }
+  __device__ __forceinline__ uint4 fixc(int c) const { return q[c]; }
 };
 
 // Relative velocity v_ij with Morris boundary conditions for wall and colloid
@@ -667,8 +675,9 @@ struct CollFromLds {
 template <class R, int DIM, bool WALL, class CV>
 __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, int ti,
                                            int tj, double xi, double yi, double zi,
-                                           double xj, double yj, double zj, const R vi[3],
-                                           const R vj[3], const R rij[3], R v[3]) {
+                                           double xj, double yj, double zj, uint4 Xi, uint4 Xj,
+                                           const R vi[3], const R vj[3], const R rij[3],
+                                           R v[3]) {
   const R bmax = K.bmax;
   if (ti == tj) {
 #pragma unroll
@@ -693,8 +702,11 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
     } else {
       const int c = tj - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B(C_X, c), B(C_Y, c),
-                             DIM == 3 ? B(C_Z, c) : 0.0, rc);
+      if constexpr (kFixRec<R, DIM>)  // (fixed-point forms: the pairs' own arithmetic)
+        fix_sep<DIM>(K.u, Xi, B.fixc(c), rc);
+      else
+        pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B(C_X, c), B(C_Y, c),
+                               DIM == 3 ? B(C_Z, c) : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R dis = dc - Rc;
@@ -733,8 +745,11 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
     } else {
       const int c = ti - 3;
       R rc[3];
-      pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B(C_X, c), B(C_Y, c),
-                             DIM == 3 ? B(C_Z, c) : 0.0, rc);
+      if constexpr (kFixRec<R, DIM>)
+        fix_sep<DIM>(K.u, Xj, B.fixc(c), rc);
+      else
+        pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B(C_X, c), B(C_Y, c),
+                               DIM == 3 ? B(C_Z, c) : 0.0, rc);
       R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R djs = dc - Rc;
@@ -1672,10 +1687,13 @@ __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<
     // (the fixed-point coordinates mapped back; used only by wall and colloid pairs)
     morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.pos(K, 0), me.pos(K, 1),
                              DIM == 3 ? me.pos(K, 2) : 0.0, pos_of(K, rj, 0), pos_of(K, rj, 1),
-                             DIM == 3 ? pos_of(K, rj, 2) : 0.0, me.v, vj, r, v);
+                             DIM == 3 ? pos_of(K, rj, 2) : 0.0,
+                             make_uint4(me.X[0], me.X[1], me.X[2], 0u),
+                             make_uint4(rj.X[0], rj.X[1], rj.X[2], 0u), me.v, vj, r, v);
   } else {
-    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, rj.x[0], rj.x[1], rj.x[2], me.v,
-                             vj, r, v);
+    morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.x, me.y, me.z, rj.x[0], rj.x[1], rj.x[2],
+                             make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u), me.v, vj,
+                             r, v);
   }
   if (on_gv) {
     // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
@@ -1859,19 +1877,23 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   const Bodies B = bodies_of(Ball, T, sim);
   // the colloid states of the block's simulation, staged before any thread leaves
   __shared__ double cst[C_OZ + 1][kCollStage];
+  __shared__ uint4 cstq[kCollStage];
   if constexpr (CST) {
     const int nc = P.N_colloids;  // (uniform: one simulation per workgroup)
     if (nc > 0) {
       // (fields unrolled: a run-time field index would put B.s in scratch)
-      for (int c = threadIdx.x; c < nc; c += blockDim.x)
+      for (int c = threadIdx.x; c < nc; c += blockDim.x) {
 #pragma unroll
         for (int f = 0; f <= C_OZ; ++f) cst[f][c] = B.s[f][c];
+        if constexpr (kFixRec<R, DIM>)
+          cstq[c] = fix_pos(fix_of(P), B.s[C_X][c], B.s[C_Y][c], B.s[C_Z][c]);
+      }
       __syncthreads();
     }
   }
   const auto cv = [&] {
-    if constexpr (CST) return CollFromLds{cst};
-    else return CollFromBodies{B};
+    if constexpr (CST) return CollFromLds{cst, cstq};
+    else return CollFromBodies{B, P};
   }();
   // Verlet walk: every lane of a wave takes part in the quad loads, so the
   // lanes past the simulation's end stay (aliasing its last particle for their
@@ -2103,7 +2125,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
         const R r2 = (R)cme.r2(c);
         if (r2 < rcutsq && k != i) {
           if (hit >= skip)
-            pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec(rec, k), CollFromBodies{B}, me, acc);
+            pair_body<R, DIM, WALL, DO_GV, DO_F>(K, load_rec(rec, k), CollFromBodies{B, P}, me,
+                                                 acc);
           hit++;
         }
       }
