@@ -303,6 +303,20 @@ __device__ __forceinline__ void pair_root(float r2, float &rr, float &ri) {
 __device__ __forceinline__ float pair_sqrt(float r2) { return sqrtf(r2); }
 #endif
 
+// Division and square root in the Morris branches of the pair pass: the FP64
+// form keeps IEEE division and sqrt; the FP32 form uses the hardware
+// reciprocal v_rcp_f32 and v_sqrt_f32 (1 ulp) unless SPH_FP32_IEEE_MATH is
+// defined, as its pair roots do (an IEEE float division is ~10 instructions).
+__device__ __forceinline__ double pair_div(double a, double b) { return a / b; }
+__device__ __forceinline__ double branch_sqrt(double a) { return sqrt(a); }
+#ifndef SPH_FP32_IEEE_MATH
+__device__ __forceinline__ float pair_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float branch_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+#else
+__device__ __forceinline__ float pair_div(float a, float b) { return a / b; }
+__device__ __forceinline__ float branch_sqrt(float a) { return sqrtf(a); }
+#endif
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8), and each XCD has its own L2.  The pair kernels map workgroup b to
 // tile xcd_tile(b) so that XCD k walks the k-th contiguous eighth of the
@@ -692,7 +706,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
       } else {
         di = (R)(K.y_top - yi); dj = (R)(yj - K.y_top); Vw = K.V_top;
       }
-      R beta = R(1) + dj / di;
+      R beta = R(1) + pair_div(dj, di);
       if (beta > bmax) beta = bmax;
       if (di > R(0)) {
         v[0] = beta * (vi[0] - Vw); v[1] = beta * vi[1]; v[2] = beta * vi[2];
@@ -707,7 +721,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
       else
         pair_sep<R, DIM, WALL>(K.box(), xi, yi, zi, B(C_X, c), B(C_Y, c),
                                DIM == 3 ? B(C_Z, c) : 0.0, rc);
-      R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
+      R dc = branch_sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R dis = dc - Rc;
       const R Vc[3] = {(R)B(C_VX, c), (R)B(C_VY, c), DIM == 3 ? (R)B(C_VZ, c) : R(0)};
@@ -715,13 +729,13 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = vi[a] - Vc[a];
       } else {
-        R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
+        R sv[3] = {pair_div(rc[0], dc), pair_div(rc[1], dc), pair_div(rc[2], dc)};
         R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
         R ox = DIM == 3 ? (R)B(C_OX, c) : R(0), oy = DIM == 3 ? (R)B(C_OY, c) : R(0);
         R oz = (R)B(C_OZ, c);
         R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
         R djs = (rij[0] * sv[0] + rij[1] * sv[1] + rij[2] * sv[2]) - dis;
-        R beta = R(1) + djs / dis;
+        R beta = R(1) + pair_div(djs, dis);
         if (beta > bmax) beta = bmax;
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = beta * (vi[a] - Vc[a] - vr[a]);
@@ -735,7 +749,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
       } else {
         di = (R)(yi - K.y_top); dj = (R)(K.y_top - yj); Vw = K.V_top;
       }
-      R beta = R(1) + di / dj;
+      R beta = R(1) + pair_div(di, dj);
       if (beta > bmax) beta = bmax;
       if (dj > R(0)) {
         v[0] = beta * (Vw - vj[0]); v[1] = -beta * vj[1]; v[2] = -beta * vj[2];
@@ -750,7 +764,7 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
       else
         pair_sep<R, DIM, WALL>(K.box(), xj, yj, zj, B(C_X, c), B(C_Y, c),
                                DIM == 3 ? B(C_Z, c) : 0.0, rc);
-      R dc = sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
+      R dc = branch_sqrt(rc[0] * rc[0] + rc[1] * rc[1] + rc[2] * rc[2]);
       const R Rc = K.coll_R;
       R djs = dc - Rc;
       const R Vc[3] = {(R)B(C_VX, c), (R)B(C_VY, c), DIM == 3 ? (R)B(C_VZ, c) : R(0)};
@@ -758,13 +772,13 @@ __device__ __forceinline__ void morris_vij(const PairK<R, DIM> &K, const CV &B, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = Vc[a] - vj[a];
       } else {
-        R sv[3] = {rc[0] / dc, rc[1] / dc, rc[2] / dc};
+        R sv[3] = {pair_div(rc[0], dc), pair_div(rc[1], dc), pair_div(rc[2], dc)};
         R xs = sv[0] * Rc, ys = sv[1] * Rc, zs = sv[2] * Rc;
         R ox = DIM == 3 ? (R)B(C_OX, c) : R(0), oy = DIM == 3 ? (R)B(C_OY, c) : R(0);
         R oz = (R)B(C_OZ, c);
         R vr[3] = {oy * zs - oz * ys, oz * xs - ox * zs, ox * ys - oy * xs};
         R dis = (-rij[0] * sv[0] - rij[1] * sv[1] - rij[2] * sv[2]) - djs;
-        R beta = R(1) + dis / djs;
+        R beta = R(1) + pair_div(dis, djs);
         if (beta > bmax) beta = bmax;
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = beta * (Vc[a] + vr[a] - vj[a]);
