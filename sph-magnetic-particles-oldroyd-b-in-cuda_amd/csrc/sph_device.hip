@@ -303,12 +303,23 @@ __device__ __forceinline__ void pair_root(float r2, float &rr, float &ri) {
 __device__ __forceinline__ float pair_sqrt(float r2) { return sqrtf(r2); }
 #endif
 
-// Division and square root in the Morris branches of the pair pass: the FP64
-// form keeps IEEE division and sqrt; the FP32 form uses the hardware
-// reciprocal v_rcp_f32 and v_sqrt_f32 (1 ulp) unless SPH_FP32_IEEE_MATH is
-// defined, as its pair roots do (an IEEE float division is ~10 instructions).
-__device__ __forceinline__ double pair_div(double a, double b) { return a / b; }
-__device__ __forceinline__ double branch_sqrt(double a) { return sqrt(a); }
+// Division and square root in the Morris branches of the pair pass (an IEEE
+// division is ~10 instructions in float, more in double).  FP64 form: the
+// hardware reciprocal refined by two Newton steps, each of which squares the
+// relative error, times the numerator (within 2 ulp), and the pair roots'
+// Newton-refined square root.  FP32 form: the hardware reciprocal v_rcp_f32
+// and v_sqrt_f32 (1 ulp) unless SPH_FP32_IEEE_MATH is defined, as its pair
+// roots do.
+// (b = 0 and a = 0 keep the IEEE results: the Newton step would turn the
+// reciprocal's infinity into 0 * inf = NaN.)
+__device__ __forceinline__ double pair_div(double a, double b) {
+  const double y0 = __builtin_amdgcn_rcp(b);
+  double y = y0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) y = fma(y, fma(-b, y, 1.0), y);
+  return a * (b == 0.0 ? y0 : y);
+}
+__device__ __forceinline__ double branch_sqrt(double a) { return a == 0.0 ? a : pair_sqrt(a); }
 #ifndef SPH_FP32_IEEE_MATH
 __device__ __forceinline__ float pair_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float branch_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
