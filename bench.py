@@ -177,7 +177,7 @@ def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
     }
 
 
-def time_pipeline(n_train, n_hold, n_sweep, steps, n=48, seed=21, log=None):
+def time_pipeline(n_train, n_hold, n_sweep, steps, n=48, seed=21, fit_steps=6000, log=None):
     """The learned sweep: SPH runs train the MLP ensemble; a new sweep is then
     answered by the ensemble, with the points whose spread of the wall stress
     exceeds 5 % re-run by batched SPH.  Compared with running the whole new
@@ -189,7 +189,7 @@ def time_pipeline(n_train, n_hold, n_sweep, steps, n=48, seed=21, log=None):
     X = surrogate.feature_matrix(train)
     model = surrogate.Surrogate(members=5, seed=seed)
     t0 = time.perf_counter()
-    model.fit(X[:n_train], Y[:n_train], steps=6000, batch=256, lr=2e-3, lr_end=1e-5)
+    model.fit(X[:n_train], Y[:n_train], steps=fit_steps, batch=256, lr=2e-3, lr_end=1e-5)
     t_fit = time.perf_counter() - t0
     mh, _ = model.predict(X[n_train:])
     ok = np.isfinite(Y[n_train:, 0])
@@ -230,6 +230,7 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=200)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--pipeline-points", type=int, default=1000)
+    ap.add_argument("--fit-steps", type=int, default=6000, help="Adam steps of the surrogate")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
@@ -252,7 +253,7 @@ def main():
                   f"at a time", file=sys.stderr, flush=True)
         pipe = None
         if not a.no_pipeline:
-            pipe = time_pipeline(1800, 200, a.pipeline_points, 800)
+            pipe = time_pipeline(1800, 200, a.pipeline_points, 800, fit_steps=a.fit_steps)
             print(f"# pipeline: x{pipe['speedup']:.1f} vs plain batched, "
                   f"{pipe['fallback_points']} of {a.pipeline_points} re-run, median wall-stress "
                   f"error {pipe['wall_stress_rel_err_median']:.3%}", file=sys.stderr, flush=True)
