@@ -233,7 +233,7 @@ def main():
     ap.add_argument("--fit-steps", type=int, default=6000, help="Adam steps of the surrogate")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
+        print("bench.py: the benchmark runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     with tempfile.TemporaryDirectory() as tmpdir:
