@@ -74,3 +74,41 @@ def test_full_size_walls_colloids(precision, tol, tmp_path):
         e = relerr(s.get(n), o.get(n))
         assert e <= tol["f_step"], f"step {n}: rel err {e:.3e} > {tol['f_step']:.0e}"
     s.close()
+
+
+def test_full_size_verlet_rebuilds_fp64(tmp_path, monkeypatch):
+    """120 steps of the same flow, FP64 form, with Verlet lists of skin 0.005
+    rcut (rebuilt every few steps near the walls, each rebuild reordering the
+    rebuilt state in place) against a cell search every step (skin 0): the
+    pairs are the same, only the summation order differs. Positions 1e-12
+    absolute; v, c, F, density 1e-10 normwise."""
+    cfg = inputs.walls_colloids_3d(n=100, n_colloids=16, seed=3)
+    names = comps("x", 3) + comps("v", 3) + comps("c", 3) + comps("f", 3) + ["dens"]
+    out = {}
+    for skin in ("0.005", "0"):
+        monkeypatch.setenv("SPH_VERLET_SKIN", skin)
+        s = helpers.make_system(cfg, str(tmp_path), sph_system.FP64, noise=1e-2,
+                                name=f"skin{skin}")
+        helpers.perturb(s)
+        print(f"full size verlet skin {skin}: N={s.N}", flush=True)
+        s.prepare()
+        x0 = [s.get(n) for n in comps("x", 3)]
+        s.step(120)
+        s.synchronize()
+        out[skin] = {n: s.get(n) for n in names}
+        L, rcut = s.params.L, s.params.rcut
+        if skin != "0":
+            # lists are rebuilt once a particle has moved skin / 2 from its
+            # position at the last build: a largest displacement of four times
+            # that means at least three rebuilds happened in these steps
+            d2 = sum(helpers.periodic_diff(out[skin][n], x0[k], L[k]) ** 2
+                     for k, n in enumerate(comps("x", 3)))
+            assert np.sqrt(d2.max()) >= 4 * 0.5 * float(skin) * rcut
+        s.close()
+    a, b = out["0.005"], out["0"]
+    for k, n in enumerate(comps("x", 3)):
+        diff = helpers.periodic_diff(a[n], b[n], L[k])
+        assert np.max(np.abs(diff)) <= 1e-12, n
+    for n in names[3:]:
+        e = relerr(a[n], b[n])
+        assert e <= 1e-10, f"{n}: rel err {e:.3e}"
