@@ -112,3 +112,42 @@ def test_full_size_verlet_rebuilds_fp64(tmp_path, monkeypatch):
     for n in names[3:]:
         e = relerr(a[n], b[n])
         assert e <= 1e-10, f"{n}: rel err {e:.3e}"
+
+
+def test_full_size_sweep_batch_members_match_single_runs(tmp_path):
+    """The batch of the sweep measurement at its full size: 256 2D wall +
+    colloid simulations, lattice side 32-160 (seed 4), 2,839,441 particles, FP32
+    form. After prepare and 20 steps (captured-graph replays with Verlet
+    rebuilds), four members - the smallest, the largest and the two with the
+    most colloids - equal their stand-alone runs bit for bit."""
+    import sweep
+
+    pts = sweep.sample_points(256, seed=4)
+    systems = [sweep.build_system(p, 200, sph_system.FP32, str(tmp_path)) for p in pts]
+    assert sum(s.N for s in systems) == 2839441
+    b = sph_system.Batch(sph_system.FP32)
+    for s in systems:
+        b.add(s)
+    sizes = [s.N for s in systems]
+    ncoll = [s.params.N_colloids for s in systems]
+    by_coll = sorted(range(len(systems)), key=lambda k: -ncoll[k])
+    pick = list(dict.fromkeys([int(np.argmin(sizes)), int(np.argmax(sizes))] + by_coll[:2]))
+    print(f"full size sweep batch: {len(systems)} sims, members {pick}", flush=True)
+    b.prepare()
+    b.step(20)
+    b.synchronize()
+    for k in pick:
+        s = systems[k]
+        s.prepare()
+        s.step(20)
+        s.synchronize()
+        p = s.params
+        names = (comps("x", 2) + comps("v", 2) + comps("c", 2) + comps("f", 2)
+                 + ["dens", "fx_wall", "fy_wall"])
+        if p.N_colloids:
+            names += ["coll_x", "coll_vx", "coll_omegaz", "fx_colloid", "tz_colloid"]
+        for n in names:
+            np.testing.assert_array_equal(b.get(k, n), s.get(n), err_msg=f"sim {k} {n}")
+    b.close()
+    for s in systems:
+        s.close()
