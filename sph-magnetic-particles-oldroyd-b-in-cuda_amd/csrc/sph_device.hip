@@ -1558,10 +1558,31 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 // checks.  Lists that overflow their
 // capacity raise overflow[sim]: those particles use the strip scan, and the
 // simulation is rebuilt again next step.
+// Step counter and magnetic moment m(t) = (cos w t, sin w t, 0) of simulation
+// s at t = t0 + dt k (src/main.cu:290-293, 395-400); `step` >= 0 sets k, -1
+// advances the device counter.
+__device__ __forceinline__ void set_step_of(const SimTable &T, int *steps, double *mag, int step,
+                                            int s) {
+  const sph_params_t &P = T.P[s];
+  const int k = step >= 0 ? step : steps[s] + 1;
+  steps[s] = k;
+  const double ang = P.omega_magnet * (P.t0 + P.dt * (double)k);
+  mag[3 * s] = cos(ang);
+  mag[3 * s + 1] = sin(ang);
+  mag[3 * s + 2] = 0.0;
+}
+
+// `steps` non-null: the first nsim threads of the grid also do k_set_step's
+// work for this step (one launch fewer per step).
 template <int DIM, bool WALL, class R>
 __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R> S,
                                                          const int *__restrict__ csg, VList VL,
-                                                         VerletCtl V) {
+                                                         VerletCtl V, int *steps, double *mag,
+                                                         int step) {
+  if (steps) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < T.nsim) set_step_of(T, steps, mag, step, s);
+  }
   const int b = tile_of(T), sim = T.blk_sim[b];
   if (!V.rebuild[sim]) return;
   const int i = T.blk_first[b] + threadIdx.x;
@@ -2593,14 +2614,7 @@ __global__ __launch_bounds__(kBlock) void k_ke_final(SimTable T, const double *p
 // counter; -1 advances it by one (the form a captured graph replays).
 __global__ void k_set_step(SimTable T, int *steps, double *mag, int step) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= T.nsim) return;
-  const sph_params_t &P = T.P[s];
-  const int k = step >= 0 ? step : steps[s] + 1;
-  steps[s] = k;
-  const double ang = P.omega_magnet * (P.t0 + P.dt * (double)k);
-  mag[3 * s] = cos(ang);
-  mag[3 * s + 1] = sin(ang);
-  mag[3 * s + 2] = 0.0;
+  if (s < T.nsim) set_step_of(T, steps, mag, step, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -3269,13 +3283,14 @@ class DeviceImpl final : public Device {
       dispatch3([&](auto dimc, auto wallc) {
         constexpr int DIM = decltype(dimc)::value;
         constexpr bool WALL = decltype(wallc)::value;
-        k_verlet_build<DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), cs_.p,
-                                                                 vlist(), vctl());
+        // with colloids, the launch also sets the step counter and m(t)
+        k_verlet_build<DIM, WALL><<<nblk_, block_, 0, stream_>>>(
+            table(), parts(cur_), cs_.p, vlist(), vctl(), ncoll_ ? steps_.p : nullptr, mag_.p,
+            step);
       });
       // (the density pass that follows turns overflows into next step's rebuilds)
       HIPCHK(hipGetLastError());
     }
-    if ((rc = set_step(step, err))) return rc;
     return forces_and_kick(err);
   }
 
