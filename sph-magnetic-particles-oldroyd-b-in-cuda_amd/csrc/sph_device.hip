@@ -2374,9 +2374,86 @@ __device__ __forceinline__ void displacement_check(const VerletCtl &V, const sph
   atomicOr(V.any, 1);
 }
 
+__device__ __forceinline__ void quat_mul(const double c[4], const double d[4], double r[4]) {
+  // kernel_quaternion_product (src/kernel_quaternion_product.cu:15-32)
+  r[0] = c[0] * d[0] - c[1] * d[1] - c[2] * d[2] - c[3] * d[3];
+  r[1] = c[0] * d[1] + c[1] * d[0] + c[2] * d[3] - c[3] * d[2];
+  r[2] = c[0] * d[2] - c[1] * d[3] + c[2] * d[0] + c[3] * d[1];
+  r[3] = c[0] * d[3] + c[1] * d[2] - c[2] * d[1] + c[3] * d[0];
+}
+
+// Colloid half-kick, drift and rotation (one thread per colloid of the batch;
+// simulations with coll_move != 0 keep their colloids fixed).  A 3D colloid
+// with |omega| == 0 keeps its orientation (the reference divides by zero).
+__device__ __forceinline__ void coll_vv1_of(const SimTable &T, Bodies B, int i) {
+  const sph_params_t &P = T.P[T.coll_sim[i]];
+  if (P.coll_move != 0) return;
+  double **C = B.s;
+  const bool d3 = P.dim == 3;
+  const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
+  C[C_VX][i] += hdtm * C[C_FX][i];
+  C[C_VY][i] += hdtm * C[C_FY][i];
+  if (d3) C[C_VZ][i] += hdtm * C[C_FZ][i];
+  C[C_OZ][i] += hdtI * C[C_TZ][i];
+  double omega;
+  if (!d3) {
+    omega = C[C_OZ][i];
+  } else {
+    C[C_OX][i] += hdtI * C[C_TX][i];
+    C[C_OY][i] += hdtI * C[C_TY][i];
+    omega = sqrt(C[C_OX][i] * C[C_OX][i] + C[C_OY][i] * C[C_OY][i] + C[C_OZ][i] * C[C_OZ][i]);
+  }
+  double cx = C[C_X][i] + C[C_VX][i] * P.dt;
+  double cy = C[C_Y][i] + C[C_VY][i] * P.dt;
+  double cz = d3 ? C[C_Z][i] + C[C_VZ][i] * P.dt : 0.0;
+  if (cx < 0) cx = cx + P.L[0];
+  if (cx > P.L[0]) cx = cx - P.L[0];
+  if (P.wall == 0) {
+    if (cy < 0) cy = cy + P.L[1];
+    if (cy > P.L[1]) cy = cy - P.L[1];
+  }
+  if (d3) {
+    if (cz < 0) cz = cz + P.L[2];
+    if (cz > P.L[2]) cz = cz - P.L[2];
+  }
+  C[C_X][i] = cx;
+  C[C_Y][i] = cy;
+  if (d3) C[C_Z][i] = cz;
+  if (!d3) {
+    C[C_THETA][i] = C[C_THETA][i] + C[C_OZ][i] * P.dt;
+  } else {
+    double h = 0.5 * omega * P.dt;
+    double dq[4] = {1.0, 0.0, 0.0, 0.0};
+    if (omega != 0.0) {
+      double sh = sin(h);
+      dq[0] = cos(h);
+      dq[1] = sh * (C[C_OX][i] / omega);
+      dq[2] = sh * (C[C_OY][i] / omega);
+      dq[3] = sh * (C[C_OZ][i] / omega);
+    }
+    double q[4] = {C[C_Q0][i], C[C_QX][i], C[C_QY][i], C[C_QZ][i]}, qn[4];
+    quat_mul(dq, q, qn);
+    C[C_Q0][i] = qn[0];
+    C[C_QX][i] = qn[1];
+    C[C_QY][i] = qn[2];
+    C[C_QZ][i] = qn[3];
+  }
+}
+
+__global__ void k_coll_vv1(SimTable T, Bodies B, int ncoll) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncoll) coll_vv1_of(T, B, i);
+}
+
 template <class R, int DIM, bool WALL>
 __global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Derived<R> D,
-                                                VerletCtl V) {
+                                                VerletCtl V, Bodies B, int ncoll) {
+  // ncoll > 0: the first ncoll threads of the grid also do k_coll_vv1's work
+  // (the colloid bodies; this kernel leaves colloid particles to k_coll_place)
+  if (ncoll) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ncoll) coll_vv1_of(T, B, g);
+  }
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
@@ -2441,73 +2518,7 @@ __global__ __launch_bounds__(kBlock) void k_vv2(SimTable T, Particles<R> S, Deri
   for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) S.c[k][i] = S.c[k][i] + hdt * D.dcdt[k][i];
 }
 
-__device__ __forceinline__ void quat_mul(const double c[4], const double d[4], double r[4]) {
-  // kernel_quaternion_product (src/kernel_quaternion_product.cu:15-32)
-  r[0] = c[0] * d[0] - c[1] * d[1] - c[2] * d[2] - c[3] * d[3];
-  r[1] = c[0] * d[1] + c[1] * d[0] + c[2] * d[3] - c[3] * d[2];
-  r[2] = c[0] * d[2] - c[1] * d[3] + c[2] * d[0] + c[3] * d[1];
-  r[3] = c[0] * d[3] + c[1] * d[2] - c[2] * d[1] + c[3] * d[0];
-}
 
-// Colloid half-kick, drift and rotation (one thread per colloid of the batch;
-// simulations with coll_move != 0 keep their colloids fixed).  A 3D colloid
-// with |omega| == 0 keeps its orientation (the reference divides by zero).
-__global__ void k_coll_vv1(SimTable T, Bodies B, int ncoll) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncoll) return;
-  const sph_params_t &P = T.P[T.coll_sim[i]];
-  if (P.coll_move != 0) return;
-  double **C = B.s;
-  const bool d3 = P.dim == 3;
-  const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
-  C[C_VX][i] += hdtm * C[C_FX][i];
-  C[C_VY][i] += hdtm * C[C_FY][i];
-  if (d3) C[C_VZ][i] += hdtm * C[C_FZ][i];
-  C[C_OZ][i] += hdtI * C[C_TZ][i];
-  double omega;
-  if (!d3) {
-    omega = C[C_OZ][i];
-  } else {
-    C[C_OX][i] += hdtI * C[C_TX][i];
-    C[C_OY][i] += hdtI * C[C_TY][i];
-    omega = sqrt(C[C_OX][i] * C[C_OX][i] + C[C_OY][i] * C[C_OY][i] + C[C_OZ][i] * C[C_OZ][i]);
-  }
-  double cx = C[C_X][i] + C[C_VX][i] * P.dt;
-  double cy = C[C_Y][i] + C[C_VY][i] * P.dt;
-  double cz = d3 ? C[C_Z][i] + C[C_VZ][i] * P.dt : 0.0;
-  if (cx < 0) cx = cx + P.L[0];
-  if (cx > P.L[0]) cx = cx - P.L[0];
-  if (P.wall == 0) {
-    if (cy < 0) cy = cy + P.L[1];
-    if (cy > P.L[1]) cy = cy - P.L[1];
-  }
-  if (d3) {
-    if (cz < 0) cz = cz + P.L[2];
-    if (cz > P.L[2]) cz = cz - P.L[2];
-  }
-  C[C_X][i] = cx;
-  C[C_Y][i] = cy;
-  if (d3) C[C_Z][i] = cz;
-  if (!d3) {
-    C[C_THETA][i] = C[C_THETA][i] + C[C_OZ][i] * P.dt;
-  } else {
-    double h = 0.5 * omega * P.dt;
-    double dq[4] = {1.0, 0.0, 0.0, 0.0};
-    if (omega != 0.0) {
-      double sh = sin(h);
-      dq[0] = cos(h);
-      dq[1] = sh * (C[C_OX][i] / omega);
-      dq[2] = sh * (C[C_OY][i] / omega);
-      dq[3] = sh * (C[C_OZ][i] / omega);
-    }
-    double q[4] = {C[C_Q0][i], C[C_QX][i], C[C_QY][i], C[C_QZ][i]}, qn[4];
-    quat_mul(dq, q, qn);
-    C[C_Q0][i] = qn[0];
-    C[C_QX][i] = qn[1];
-    C[C_QY][i] = qn[2];
-    C[C_QZ][i] = qn[3];
-  }
-}
 
 // Rigidly re-places every colloid boundary particle (one thread per list entry).
 template <class R>
@@ -3209,13 +3220,16 @@ class DeviceImpl final : public Device {
     list_valid_ = false;
     rec_valid_ = false;
     const VerletCtl V = vctl();
+    // the particle launch also moves the colloids when its grid has a thread each
+    const bool fuse = (long long)nblk_ * block_ >= ncoll_;
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
-      k_vv1<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(), V);
+      k_vv1<R, DIM, WALL><<<nblk_, block_, 0, stream_>>>(table(), parts(cur_), derived(), V,
+                                                           bodies(), fuse ? ncoll_ : 0);
     });
     if (ncoll_ > 0) {
-      k_coll_vv1<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
+      if (!fuse) k_coll_vv1<<<blocks(ncoll_), kBlock, 0, stream_>>>(table(), bodies(), ncoll_);
       if (nclist_ > 0)
         k_coll_place<R><<<blocks(nclist_), kBlock, 0, stream_>>>(table(), parts(cur_), bodies(),
                                                                  rank_.p, nclist_, V);
