@@ -961,7 +961,10 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *t
       }
     }
   }
-  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = total;
+    if (gridDim.x == 1) cs[ncells] = total;  // one tile: the scan ends here
+  }
 }
 
 // One workgroup: tile totals -> exclusive tile offsets (at most kScanTile
@@ -988,6 +991,27 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, i
   if (any && !*any) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncells && c >= kScanTile) cs[c] += tsum[c / kScanTile];
+}
+
+// Up to kScanSumTiles tiles, k_scan_add_sums replaces k_scan_tile_sums +
+// k_scan_add: each cell adds the totals of the tiles before its own, and the
+// first thread writes the grand total into cs[ncells].
+constexpr int kScanSumTiles = 64;
+__global__ __launch_bounds__(kBlock) void k_scan_add_sums(int *cs, const int *tsum, int ntiles,
+                                                          int ncells, const int *any) {
+  if (any && !*any) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) {
+    int t = 0;
+    for (int k = 0; k < ntiles; ++k) t += tsum[k];
+    cs[ncells] = t;
+  }
+  if (c < ncells && c >= kScanTile) {
+    const int tile = c / kScanTile;
+    int off = 0;
+    for (int k = 0; k < tile; ++k) off += tsum[k];
+    cs[c] += off;
+  }
 }
 
 // rebuild (when not null): only simulations with rebuild[s] set are reordered.
@@ -3049,9 +3073,13 @@ class DeviceImpl final : public Device {
                                           flag_.p, any);
     const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
     k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_, any);
-    k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_, any);
-    if (ntiles > 1)
+    if (ntiles > kScanSumTiles) {
+      k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_, any);
       k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_, any);
+    } else if (ntiles > 1) {
+      k_scan_add_sums<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ntiles, ncells_,
+                                                               any);
+    }
     k_scatter<<<nblk_, block_, 0, stream_>>>(table(), cell_.p, slot_.p, cs_.p, S.id,
                                              stage_src_.p, stage_id_.p, gate, any);
     k_rank_gather<R><<<nblk_, block_, 0, stream_>>>(table(), S, T, cell_.p, cs_.p, stage_src_.p,
