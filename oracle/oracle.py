@@ -4,6 +4,7 @@ Only tests/, the smoke check and the bench's CPU-baseline leg use this module,
 and only as the checker.  Field names and lengths are those of sph.h.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -37,8 +38,9 @@ def library():
     for fn in ("orc_set_field", "orc_get_field"):
         getattr(lib, fn).restype = i32
         getattr(lib, fn).argtypes = [vp, ctypes.c_char_p, vp, i64]
-    lib.orc_get_cell_list.restype = i32
-    lib.orc_get_cell_list.argtypes = [vp, vp, vp, vp, vp]
+    for fn in ("orc_get_cell_list", "orc_get_colloid_cell_list"):
+        getattr(lib, fn).restype = i32
+        getattr(lib, fn).argtypes = [vp, vp, vp, vp, vp]
     for fn in ("orc_neighbours_search", "orc_calculate_density", "orc_calculate_vel_gradient",
                "orc_calculate_pressure", "orc_calculate_forces", "orc_V_Verlet_step1",
                "orc_V_Verlet_step2", "orc_prepare", "orc_calculate_pair_forces",
@@ -48,6 +50,8 @@ def library():
         getattr(lib, fn).argtypes = [vp]
     lib.orc_set_step.restype = i32
     lib.orc_set_step.argtypes = [vp, i32]
+    lib.orc_set_slip.restype = i32
+    lib.orc_set_slip.argtypes = [vp, i32]
     lib.orc_step.restype = i32
     lib.orc_step.argtypes = [vp, i32]
     lib.orc_kinetic_energy.restype = ctypes.c_double
@@ -81,7 +85,141 @@ COLLOID_FIELDS = [
     "coll_omegax", "coll_omegay", "coll_omegaz", "coll_theta",
     "coll_quat0", "coll_quatx", "coll_quaty", "coll_quatz",
     "fx_colloid", "fy_colloid", "fz_colloid", "tx_colloid", "ty_colloid", "tz_colloid"]
+ORC_SLIP_EOS_NO_MASS = 1
+
 INT_FIELDS = {"type", "walls_list", "walls_start", "colloids_list", "colloids_start"}
+
+
+# Raw parameters: the values read from the input file (src/system_read_input.cpp)
+# plus the sizes of the arrays the set-up produced.  Everything else in
+# sph_params_t is derived from these by derive_params.
+RAW_FIELDS = (
+    "dim", "N", "Nmax", "wall", "N_colloids", "Nlist_walls", "Nlist_colloids",
+    "ext_force_type", "coll_move", "new_sim", "Nsteps",
+    "freq_micro", "freq_macro", "freq_walls", "freq_colloids",
+    "L", "dt", "t0", "overlap", "rho", "c", "P0", "eta", "zeta", "np", "kT", "tau",
+    "ext_force", "V_top", "V_bottom", "coll_R", "coll_rho", "coll_rep_cuton", "F0_rep",
+    "tau_rep", "r0_magnet", "F0_magnet", "omega_magnet")
+
+# Planted mistakes for the tests that show the pins catch them (never used
+# otherwise): each replaces one derived constant by a plausible wrong formula.
+SLIPS = ("csq_c", "rho0_sum", "nkT_sum", "a_no_zeta", "b_2d_form", "cw_other_dim",
+         "c_gradw_sign", "rcut_no_overlap", "coll_I_disc", "rep_cutoff_no_tau")
+
+
+def derive_params(src, slip=None):
+    """Every derived field of sph_params_t recomputed from the raw ones of `src`
+    by the reference's formulas, in the reference's arithmetic order
+    (src/system_initialize.cpp:47-262, src/system_constructor.cpp:61-78).
+    The oracle runs on these, so the checker shares no host arithmetic with the
+    system under test.  `src.Nxyz[1]` is taken as the system set it (lattice
+    rows including the wall layers, src/system_initialize.cpp:76-80)."""
+    from sph_system import SphParams
+
+    p = SphParams()
+    for name in RAW_FIELDS:
+        v = getattr(src, name)
+        if hasattr(v, "_length_"):
+            for k in range(len(v)):
+                getattr(p, name)[k] = v[k]
+        else:
+            setattr(p, name, v)
+    dim, wall, overlap = p.dim, p.wall, p.overlap
+    L = list(p.L)
+    wall_layers = 2 * math.ceil(overlap) if wall == 1 else 0
+    N0, N1, N2 = src.Nxyz[0], src.Nxyz[1] - wall_layers, src.Nxyz[2]
+    p.Nwalls = 2 if wall else 0                                    # system_constructor.cpp
+    p.nkT = p.np * p.kT                                            # :53
+    if slip == "nkT_sum":
+        p.nkT = p.np + p.kT
+    p.rcut = L[0] / N0 * overlap                                   # :56
+    if slip == "rcut_no_overlap":
+        p.rcut = L[0] / N0
+    p.rcutsq = p.rcut * p.rcut                                     # :57
+    p.rcut_inv = 1.0 / p.rcut                                      # :58
+    dx = [L[0] / N0, L[1] / N1, (L[2] / N2) if dim == 3 else 0.0]  # :61-63
+    p.ky = 2.0 * math.pi / L[1]                                    # :66
+    eps = 1.0e-10                                                  # :70-74
+    if abs(overlap - math.floor(overlap)) < eps:
+        p.wall_width = math.floor(overlap) * dx[1]
+    else:
+        p.wall_width = (math.floor(overlap) + 1) * dx[1]
+    p.Nxyz[0], p.Nxyz[1], p.Nxyz[2] = N0, N1 + wall_layers, N2     # :76-80
+    p.y_bottom, p.y_top = (0.0, L[1]) if wall == 1 else (0.0, 0.0)
+    cw_2d = (dim == 2) != (slip == "cw_other_dim")
+    if cw_2d:                                                      # :82-85
+        p.cw = 63.0 / (478.0 * math.pi * p.rcut * p.rcut)
+    else:
+        p.cw = 27.0 / (120.0 * math.pi * p.rcut * p.rcut * p.rcut)
+    p.c_gradw = -15.0 * p.cw / p.rcut                              # :86
+    if slip == "c_gradw_sign":
+        p.c_gradw = -p.c_gradw
+    p.csq = p.c * p.c                                              # :91
+    if slip == "csq_c":
+        p.csq = p.c
+    p.rho0 = p.P0 * p.rho                                          # :93
+    if slip == "rho0_sum":
+        p.rho0 = p.P0 + p.rho
+    p.a = 5.0 * p.eta / 3.0 - p.zeta                               # :94-101
+    if slip == "a_no_zeta":
+        p.a = 5.0 * p.eta / 3.0
+    b_factor = 5.0 if dim == 3 else 4.0
+    if slip == "b_2d_form":
+        b_factor = 4.0 if dim == 3 else 5.0
+    p.b = b_factor * (p.zeta + p.eta / 3.0)
+    p.beta_max = 1.5                                               # :105
+    p.Ncells[0] = int(L[0] / p.rcut)                               # :108-125
+    p.cell_size[0] = L[0] / p.Ncells[0]
+    ly = L[1] + 2 * p.wall_width if wall == 1 else L[1]
+    p.Ncells[1] = int(ly / p.rcut)
+    p.cell_size[1] = ly / p.Ncells[1]
+    if dim == 3:
+        p.Ncells[2] = int(L[2] / p.rcut)
+        p.cell_size[2] = L[2] / p.Ncells[2]
+    else:
+        p.Ncells[2] = 1
+        p.cell_size[2] = 1
+    p.Ntotal_cells = p.Ncells[0] * p.Ncells[1] * p.Ncells[2]
+    R = p.coll_R
+    p.coll_rep_cutoff = 5.0 * R / p.tau_rep                        # :161
+    if slip == "rep_cutoff_no_tau":
+        p.coll_rep_cutoff = 5.0 * R
+    p.r0_magnet_sq = p.r0_magnet * p.r0_magnet                     # :163
+    if dim == 2:                                                   # :167-174
+        p.coll_mass = math.pi * R * R * p.coll_rho
+        p.coll_I = 0.5 * p.coll_mass * R * R
+    else:
+        p.coll_mass = 4.0 / 3.0 * math.pi * R * R * R * p.coll_rho
+        p.coll_I = 2.0 / 5.0 * p.coll_mass * R * R
+        if slip == "coll_I_disc":
+            p.coll_I = 0.5 * p.coll_mass * R * R
+    if p.N_colloids > 0:                                           # :218-242
+        p.rcuton_coll_rep = 2.0 * R + p.coll_rep_cuton
+        p.rcuton_coll_rep_sq = p.rcuton_coll_rep * p.rcuton_coll_rep
+        p.rcutoff_coll_rep = 2.0 * R + p.coll_rep_cutoff
+        p.rcutoff_coll_rep_sq = p.rcutoff_coll_rep * p.rcutoff_coll_rep
+        p.rcutoff_coll = p.rcutoff_coll_rep if p.rcutoff_coll_rep > p.r0_magnet else p.r0_magnet
+        p.rcutoff_coll_sq = p.rcutoff_coll * p.rcutoff_coll
+        for a in range(3):
+            if a < dim:
+                p.Ncells_colloids[a] = int(L[a] / p.rcutoff_coll)
+                p.cell_colloids_size[a] = L[a] / p.Ncells_colloids[a]
+            else:
+                p.Ncells_colloids[a] = 1
+                p.cell_colloids_size[a] = 1
+        p.Ntotal_cells_colloids = (p.Ncells_colloids[0] * p.Ncells_colloids[1]
+                                   * p.Ncells_colloids[2])
+    return p
+
+
+# Fields derive_params computes (compared with the system's in the tests).
+DERIVED_FIELDS = (
+    "Nwalls", "Nxyz", "Ncells", "Ntotal_cells", "Ncells_colloids", "Ntotal_cells_colloids",
+    "cell_size", "cell_colloids_size", "csq", "rho0", "a", "b", "nkT", "rcut", "rcutsq",
+    "rcut_inv", "cw", "c_gradw", "ky", "wall_width", "y_bottom", "y_top", "beta_max",
+    "coll_mass", "coll_I", "coll_rep_cutoff", "rcuton_coll_rep", "rcuton_coll_rep_sq",
+    "rcutoff_coll_rep", "rcutoff_coll_rep_sq", "rcutoff_coll", "rcutoff_coll_sq",
+    "r0_magnet_sq")
 
 
 class Oracle:
@@ -91,9 +229,11 @@ class Oracle:
         self.h = self.lib.orc_create(ctypes.byref(params))
 
     @classmethod
-    def from_system(cls, system):
-        """Oracle with the parameters and current state of a sph_system.System."""
-        o = cls(system.params)
+    def from_system(cls, system, slip=None):
+        """Oracle with the current state of a sph_system.System.  Its
+        parameters are the system's raw input values with every derived
+        constant recomputed here (derive_params), not copied."""
+        o = cls(derive_params(system.params, slip=slip))
         for n in STATE_FIELDS + SETUP_FIELDS:
             o.set(n, system.get(n))
         if system.params.N_colloids:
@@ -146,6 +286,16 @@ class Oracle:
         self.lib.orc_get_cell_list(self.h, _ptr(pc), _ptr(pi), _ptr(cs), _ptr(ce))
         return pc, pi, cs, ce
 
+    def colloid_cell_list(self):
+        p = self.params
+        nc, ncc = p.N_colloids, max(p.Ntotal_cells_colloids, 0)
+        cc = np.zeros(nc, np.int32)
+        ci = np.zeros(nc, np.int32)
+        cs = np.zeros(ncc, np.int32)
+        ce = np.zeros(ncc, np.int32)
+        self.lib.orc_get_colloid_cell_list(self.h, _ptr(cc), _ptr(ci), _ptr(cs), _ptr(ce))
+        return cc, ci, cs, ce
+
     def _call(self, fn):
         rc = getattr(self.lib, fn)(self.h)
         if rc:
@@ -197,6 +347,11 @@ class Oracle:
         rc = self.lib.orc_step(self.h, int(n))
         if rc:
             raise RuntimeError(f"orc_step returned {rc}")
+
+    def set_slip(self, code):
+        """Plants a mistake in the oracle's arithmetic (ORC_SLIP_* of
+        sph_oracle.h) for a test that shows a pin catches it."""
+        self.lib.orc_set_slip(self.h, int(code))
 
     def kinetic_energy(self):
         return self.lib.orc_kinetic_energy(self.h)

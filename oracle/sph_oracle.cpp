@@ -63,6 +63,7 @@ struct orc_system {
   std::vector<int32_t> coll_cell, coll_index, coll_cell_start, coll_cell_end;
   double mag[3];
   int step;
+  int slip;  // a planted mistake (orc_set_slip), 0 = none
 };
 
 namespace {
@@ -337,6 +338,16 @@ int orc_get_cell_list(orc_system *s, int32_t *pc, int32_t *pi, int32_t *cs, int3
   return SPH_OK;
 }
 
+int orc_get_colloid_cell_list(orc_system *s, int32_t *cc, int32_t *ci, int32_t *cs,
+                              int32_t *ce) {
+  const int Nc = s->p.N_colloids, NCC = std::max(s->p.Ntotal_cells_colloids, 0);
+  if (cc) std::memcpy(cc, s->coll_cell.data(), Nc * sizeof(int32_t));
+  if (ci) std::memcpy(ci, s->coll_index.data(), Nc * sizeof(int32_t));
+  if (cs) std::memcpy(cs, s->coll_cell_start.data(), NCC * sizeof(int32_t));
+  if (ce) std::memcpy(ce, s->coll_cell_end.data(), NCC * sizeof(int32_t));
+  return SPH_OK;
+}
+
 // neighbours_search (src/system_neighbours_search.cu:19-110): hash, stable sort
 // by (cell, ID) - Thrust's int-key radix sort is stable and starts from
 // particle_index[i] = i -, ranges; then the colloid grid when N_colloids >= 2.
@@ -446,9 +457,15 @@ int orc_calculate_vel_gradient(orc_system *s) {
 // kernel_pressures (src/kernel_pressures.cu:13-23)
 int orc_calculate_pressure(orc_system *s) {
   const sph_params_t &p = s->p;
+  const bool no_mass = s->slip == ORC_SLIP_EOS_NO_MASS;
 #pragma omp parallel for
   for (int i = 0; i < p.N; ++i)
-    s->f[PRESS][i] = p.csq * (s->f[MASS][i] * s->f[DENS][i] - p.rho0);
+    s->f[PRESS][i] = p.csq * ((no_mass ? 1.0 : s->f[MASS][i]) * s->f[DENS][i] - p.rho0);
+  return SPH_OK;
+}
+
+int orc_set_slip(orc_system *s, int slip) {
+  s->slip = slip;
   return SPH_OK;
 }
 

@@ -28,6 +28,12 @@ int orc_get_field(orc_system *s, const char *name, void *dst, int64_t count);
 int orc_get_cell_list(orc_system *s, int32_t *particle_cell, int32_t *particle_index,
                       int32_t *cell_start, int32_t *cell_end);
 
+/* Colloid cell list of the last neighbours_search (N_colloids >= 2): sorted
+ * colloid cells and colloid indices (length N_colloids), cell start / end
+ * (length Ntotal_cells_colloids; an empty cell has -1 / -2,
+ * src/kernel_initialize_cell_colloids.cu:15-25). */
+int orc_get_colloid_cell_list(orc_system *s, int32_t *coll_cell, int32_t *coll_index,
+                              int32_t *coll_cell_start, int32_t *coll_cell_end);
 int orc_neighbours_search(orc_system *s);
 int orc_calculate_density(orc_system *s);
 int orc_calculate_vel_gradient(orc_system *s);
@@ -50,6 +56,12 @@ double orc_kinetic_energy(orc_system *s);
 /* The sweep observables, defined as for sph_observables in sph.h. */
 int orc_observables(orc_system *s, double out[8]);
 int orc_num_threads(void);
+
+/* A planted mistake, for the tests that show a pin catches it (never set
+ * otherwise): ORC_SLIP_EOS_NO_MASS drops the particle mass from the equation
+ * of state, p = c^2 (d - rho0).  0 removes it. */
+enum { ORC_SLIP_NONE = 0, ORC_SLIP_EOS_NO_MASS = 1 };
+int orc_set_slip(orc_system *s, int slip);
 
 /* Stand-alone pieces for closed-form checks. */
 double orc_kernel_W(const sph_params_t *p, double r);

@@ -43,6 +43,7 @@
 #include <numeric>
 #include <vector>
 
+#include "sph_cells.h"
 #include "sph_device.h"
 
 // Multiply-add contraction only inside one source expression (a * b + c in a
@@ -843,175 +844,13 @@ __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__res
   const int i = T.blk_first[b] + threadIdx.x;
   int c = -2;  // lanes past the simulation's last particle
   if (i < T.pbeg[sim + 1]) {
-    int cx = (int)floor(x[i] / P.cell_size[0]);
-    if (cx == P.Ncells[0]) cx = cx - 1;
-    int cy;
-    if (P.wall == 0) {
-      cy = (int)floor(y[i] / P.cell_size[1]);
-      if (cy == P.Ncells[1]) cy = cy - 1;
-    } else {
-      cy = (int)floor((y[i] + P.wall_width) / P.cell_size[1]);
-    }
-    int cz = 0;
-    if (P.dim == 3) {
-      cz = (int)floor(z[i] / P.cell_size[2]);
-      if (cz == P.Ncells[2]) cz = cz - 1;
-    }
-    const bool ok = cx >= 0 && cx < P.Ncells[0] && cy >= 0 && cy < P.Ncells[1] &&
-                    (P.dim != 3 || (cz >= 0 && cz < P.Ncells[2]));
-    if (!ok) {  // also catches NaN positions (any int the conversion gives)
-      cx = min(max(cx, 0), P.Ncells[0] - 1);
-      cy = min(max(cy, 0), P.Ncells[1] - 1);
-      cz = P.dim == 3 ? min(max(cz, 0), P.Ncells[2] - 1) : 0;
-      atomicOr(&flag[sim], 1);
-    }
-    c = T.cell_off[sim] + (cz * P.Ncells[1] + cy) * P.Ncells[0] + cx;
+    bool ok;
+    c = T.cell_off[sim] + cell_of(P, x[i], y[i], z[i], ok);
+    if (!ok) atomicOr(&flag[sim], 1);
     cell[i] = c;
   }
-  // Particles arrive in the previous step's cell order, so neighbouring lanes
-  // mostly share a cell: one atomicAdd per run of equal cells in the wavefront,
-  // each lane's slot = run base + its position in the run.
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(c, 1, 64);
-  const bool head = lane == 0 || prev != c;
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  const int leader = 63 - __clzll(heads & upto);
-  const unsigned long long after = heads & ~upto;
-  const int run_end = after ? __ffsll((long long)after) - 1 : 64;
-  int base = 0;
-  if (lane == leader && c >= 0) base = atomicAdd(&count[c], run_end - leader);
-  base = __shfl(base, leader, 64);
-  if (c >= 0) slot[i] = base + (lane - leader);
-}
-
-// Exclusive scan of the per-cell counts into cs[0..ncells] in three launches:
-// each workgroup scans one tile of kScanTile cells (zeroing the counts for the
-// next hash), one workgroup scans the tile totals, and a last pass adds each
-// tile's offset.  Every workgroup has 1024 threads owning 8 consecutive values.
-constexpr int kScanTile = 8192;
-
-// Exclusive scan of v[0..8) per thread across the workgroup; returns this
-// thread's exclusive prefix and the workgroup total in *total.
-__device__ __forceinline__ int block_scan8(const int (&v)[8], int *total) {
-  __shared__ int wsum[16];
-  __shared__ int tot;
-  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  int local = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) local += v[k];
-  int incl = local;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int n = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += n;
-  }
-  if (lane == 63) wsum[wid] = incl;
-  __syncthreads();
-  if (t == 0) {
-    int acc = 0;
-    for (int w = 0; w < 16; ++w) {
-      int s = wsum[w];
-      wsum[w] = acc;
-      acc += s;
-    }
-    tot = acc;
-  }
-  __syncthreads();
-  *total = tot;
-  return wsum[wid] + incl - local;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_tiles(int *count, int *cs, int *tsum, int ncells,
-                                                     const int *any) {
-  if (any && !*any) return;
-  const int c0 = blockIdx.x * kScanTile + threadIdx.x * 8;
-  // a thread's 8 cells as two 16-byte accesses (c0 is a multiple of 8 ints)
-  // where all 8 exist, element by element at the end of the array
-  const bool whole = c0 + 8 <= ncells;
-  int v[8];
-  if (whole) {
-    const int4 a = *reinterpret_cast<const int4 *>(count + c0);
-    const int4 b = *reinterpret_cast<const int4 *>(count + c0 + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = c0 + k < ncells ? count[c0 + k] : 0;
-  }
-  int total;
-  int run = block_scan8(v, &total);
-  int o[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    o[k] = run;
-    run += v[k];
-  }
-  if (whole) {
-    *reinterpret_cast<int4 *>(cs + c0) = make_int4(o[0], o[1], o[2], o[3]);
-    *reinterpret_cast<int4 *>(cs + c0 + 4) = make_int4(o[4], o[5], o[6], o[7]);
-    *reinterpret_cast<int4 *>(count + c0) = make_int4(0, 0, 0, 0);
-    *reinterpret_cast<int4 *>(count + c0 + 4) = make_int4(0, 0, 0, 0);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (c0 + k < ncells) {
-        cs[c0 + k] = o[k];
-        count[c0 + k] = 0;
-      }
-    }
-  }
-  if (threadIdx.x == 0) {
-    tsum[blockIdx.x] = total;
-    if (gridDim.x == 1) cs[ncells] = total;  // one tile: the scan ends here
-  }
-}
-
-// One workgroup: tile totals -> exclusive tile offsets (at most kScanTile
-// tiles), and the grand total into cs[ncells].
-__global__ __launch_bounds__(1024) void k_scan_tile_sums(int *tsum, int ntiles, int *cs,
-                                                         int ncells, const int *any) {
-  if (any && !*any) return;
-  const int c0 = threadIdx.x * 8;
-  int v[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = c0 + k < ntiles ? tsum[c0 + k] : 0;
-  int total;
-  int run = block_scan8(v, &total);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (c0 + k < ntiles) tsum[c0 + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 0) cs[ncells] = total;
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_add(int *cs, const int *tsum, int ncells,
-                                                     const int *any) {
-  if (any && !*any) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncells && c >= kScanTile) cs[c] += tsum[c / kScanTile];
-}
-
-// Up to kScanSumTiles tiles, k_scan_add_sums replaces k_scan_tile_sums +
-// k_scan_add: each cell adds the totals of the tiles before its own, and the
-// first thread writes the grand total into cs[ncells].
-constexpr int kScanSumTiles = 64;
-__global__ __launch_bounds__(kBlock) void k_scan_add_sums(int *cs, const int *tsum, int ntiles,
-                                                          int ncells, const int *any) {
-  if (any && !*any) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c == 0) {
-    int t = 0;
-    for (int k = 0; k < ntiles; ++k) t += tsum[k];
-    cs[ncells] = t;
-  }
-  if (c < ncells && c >= kScanTile) {
-    const int tile = c / kScanTile;
-    int off = 0;
-    for (int k = 0; k < tile; ++k) off += tsum[k];
-    cs[c] += off;
-  }
+  const int s = count_into_cell(c, count);
+  if (c >= 0) slot[i] = s;
 }
 
 // rebuild (when not null): only simulations with rebuild[s] set are reordered.
@@ -2138,12 +1977,15 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     }
   }
   if (total > cnt) {
-    // park partial sums; k_pairs_tail continues from them
+    // Park the partial sums this pass computes (and only those: a gradient-only
+    // pass leaves the forces of an earlier force pass alone, and a force-only
+    // pass the gradient); k_pairs_tail continues from them.
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      D.f[a][i] = acc.f[a];
+      if (DO_F) D.f[a][i] = acc.f[a];
 #pragma unroll
-      for (int b = 0; b < 3; ++b) D.gv[a * 3 + b][i] = acc.g[a][b];
+      for (int b = 0; b < 3; ++b)
+        if (DO_GV) D.gv[a * 3 + b][i] = acc.g[a][b];
     }
     L.queue[atomicAdd(L.qlen, 1)] = i;
     return;
@@ -2174,9 +2016,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
     PairAcc<R> acc;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      acc.f[a] = D.f[a][i];
+      acc.f[a] = DO_F ? D.f[a][i] : R(0);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) acc.g[a][b] = D.gv[a * 3 + b][i];
+      for (int b = 0; b < 3; ++b) acc.g[a][b] = DO_GV ? D.gv[a * 3 + b][i] : R(0);
     }
     const R rcutsq = (R)P.rcutsq;
     const PairK<R, DIM> K(P);
@@ -2742,6 +2584,10 @@ template <class R>
 class DeviceImpl final : public Device {
  public:
   ~DeviceImpl() override {
+    // Queued work (sph_step returns with graph launches in flight) must end
+    // before the graph execs that hold its kernel arguments, the buffers it
+    // reads and the stream it runs on are released.
+    if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graphs();
     for (auto &e : events_) {
       (void)hipEventDestroy(e.a);
@@ -2819,6 +2665,12 @@ class DeviceImpl final : public Device {
       err = "too many cells in the batch (at most 8192 * 8192)";
       return SPH_ERR_ARG;
     }
+    // Tile offsets are added by k_scan_add_sums up to this many tiles, above it
+    // by k_scan_tile_sums + k_scan_add (SPH_SCAN_SUM_TILES lowers the threshold,
+    // so that small cell grids can take the second path too; same results).
+    scan_sum_tiles_ = kScanSumTiles;
+    if (const char *e = std::getenv("SPH_SCAN_SUM_TILES"))
+      scan_sum_tiles_ = std::min(kScanSumTiles, std::max(1, std::atoi(e)));
     HIPCHK(count_.alloc(ncells_)); HIPCHK(cs_.alloc(ncells_ + 1));
     HIPCHK(tsum_.alloc((ncells_ + kScanTile - 1) / kScanTile));
     HIPCHK(flag_.alloc(S));
@@ -2878,6 +2730,7 @@ class DeviceImpl final : public Device {
     HIPCHK(nl_queue_.alloc(N_));
     HIPCHK(nl_qlen_.alloc(1));
     HIPCHK(hipMemset(nl_qlen_.p, 0, sizeof(int)));
+    qlen_clear_ = true;
     HIPCHK(dinv_.alloc(N_));
     HIPCHK(rec_.alloc((size_t)N_ * sizeof(PairRec<R, 3>)));  // the larger record
     HIPCHK(ke_part_.alloc((size_t)kKeBlocks * S)); HIPCHK(ke_out_.alloc(S));
@@ -3073,7 +2926,7 @@ class DeviceImpl final : public Device {
                                           flag_.p, any);
     const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
     k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_, any);
-    if (ntiles > kScanSumTiles) {
+    if (ntiles > scan_sum_tiles_) {
       k_scan_tile_sums<<<1, 1024, 0, stream_>>>(tsum_.p, ntiles, cs_.p, ncells_, any);
       k_scan_add<<<blocks(ncells_), kBlock, 0, stream_>>>(cs_.p, tsum_.p, ncells_, any);
     } else if (ntiles > 1) {
@@ -3137,6 +2990,7 @@ class DeviceImpl final : public Device {
     });
     HIPCHK(hipGetLastError());
     list_valid_ = true;
+    qlen_clear_ = true;  // k_density empties the overflow queue of the next pair pass
     if (write_dens) rec_valid_ = true;
     return SPH_OK;
   }
@@ -3168,6 +3022,12 @@ class DeviceImpl final : public Device {
     // those changed since the last density/pressure pass.
     if (!rec_valid_ && (rc = pressure(err))) return rc;
     Mark m(this, phase);
+    // Each pair pass starts with an empty overflow queue.  The density pass
+    // empties it; a pair pass that reuses a list without one (e.g.
+    // calculate_vel_gradient after calculate_forces) empties it first, or its
+    // k_pairs_tail would take the earlier pass's particles a second time.
+    if (!qlen_clear_) k_fill<<<1, 64, 0, stream_>>>(nl_qlen_.p, 1, 0);
+    qlen_clear_ = false;
     dispatch3([&](auto dimc, auto wallc) {
       constexpr int DIM = decltype(dimc)::value;
       constexpr bool WALL = decltype(wallc)::value;
@@ -3785,6 +3645,7 @@ class DeviceImpl final : public Device {
   bool any_walls_ = false;
   bool coll_lds_ = true;  // every simulation's colloids fit the pair pass's LDS stage
   int nc_ = 3, cur_ = 0;
+  int scan_sum_tiles_ = kScanSumTiles;
   int block_ = kBlock, xcd_ = 1, nblk_ = 0, max_parts_ = 1;
   // Verlet lists (used by run_steps): enabled when skin > 0 and every periodic
   // axis of every simulation has at least 5 cells.
@@ -3820,6 +3681,7 @@ class DeviceImpl final : public Device {
   DevBuf<int> nl_count_, nl_queue_, nl_qlen_;
   int kmax_ = 0;
   bool list_valid_ = false;
+  bool qlen_clear_ = true;  // the overflow queue is empty for the next pair pass
   DevBuf<double> ke_part_, ke_out_, obs_part_, obs_out_;
   DevBuf<double> coll_[C_NUM], xc_, yc_, zc_, fwall_, mag_;
   DevBuf<double> wpart_;   // wall-force partial sums, 3 per chunk (k_walls)

@@ -2,6 +2,8 @@
 
 Each check is chosen so that a plausible slip in the oracle (a dropped term, a
 wrong sign, a transposed index, a wrong constant) fails it:
+  derived constants    tests/test_derived_constants.py (the oracle derives them
+                       itself from the raw inputs; EOS closed form, polymer modulus)
   kernel W, gradW      normalisation, W(0), continuity, W(rcut) = 0, gradW = dW/dr
   cell list            numpy stable argsort of the closed-form cell index
   density              O(N^2) brute force; analytic lattice sum on a perfect lattice
@@ -41,7 +43,7 @@ def _sys(cfg, tmp_path, noise=0.0, seed=1):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_kernel_normalisation_and_shape(dim, tmp_path):
     cfg = helpers.cfg_couette(n=32) if dim == 2 else helpers.cfg_sine(n=16)
-    p = _sys(cfg, tmp_path).params
+    p = oracle.derive_params(_sys(cfg, tmp_path).params)   # the oracle's cw, c_gradw
     rc = p.rcut
     W = lambda r: oracle.kernel_W(p, r)
     gW = lambda r: oracle.kernel_gradW(p, r)
@@ -83,6 +85,7 @@ def test_cell_list_closed_form(make, tmp_path):
         s.set(n, v)
         pos.append(v)
     o = oracle.Oracle.from_system(s)
+    p = o.params   # the oracle's cell grid
     o.neighbours_search()
     pc, pi, cs, ce = o.cell_list()
     # closed form of kernel_cell_list (IEEE division then floor)
