@@ -37,7 +37,8 @@ enum {
   SPH_ERR_DEVICE = 5,       /* HIP runtime error or no GPU                          */
   SPH_ERR_OUT_OF_GRID = 6,  /* a particle left the cell grid (reference: assert)    */
   SPH_ERR_NAN = 7,          /* non-finite kinetic energy                            */
-  SPH_ERR_STATE = 8         /* call out of order (e.g. step before initialize)      */
+  SPH_ERR_STATE = 8,        /* call out of order (e.g. step before initialize)      */
+  SPH_ERR_CAPACITY = 9      /* larger than sph_neighbours_search_noalloc accepts    */
 };
 
 /* ---- precision forms ---------------------------------------------------------
@@ -251,6 +252,46 @@ int sph_batch_get_launch(sph_batch_handle b, int *block, int *xcd);
  * is the one an untuned run would follow. */
 int sph_batch_autotune(sph_batch_handle b, int steps, int *block, int *xcd,
                        double *ms_per_step);
+
+/* ---- allocation-free neighbour search ------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory - code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the cell list of sph_neighbours_search (hash, stable
+ * (cell, ID) sort, cell ranges) and, when N_colloids >= 2, the reference's
+ * colloid cell grid, from positions in device memory into outputs in device
+ * memory - the data of class_system::neighbours_search without the launch
+ * geometry.  It allocates nothing: its intermediate arrays live in one static
+ * device array per GPU (9.3 MB, held until the process ends) and it calls no
+ * hipMalloc / hipFree.
+ *   p        parameters of an initialized system (sph_get_params); N, the
+ *            cell grid and the colloid grid must fit the limits below, else
+ *            SPH_ERR_CAPACITY is returned and nothing is launched
+ *   x y z    device positions in ID order (z may be NULL in 2D)
+ *   particle_cell, particle_index [N], cell_start, cell_end [Ntotal_cells]:
+ *            device outputs, the layout of sph_get_cell_list (an empty cell
+ *            has start > end)
+ *   coll_x coll_y coll_z [N_colloids] device colloid centres; coll_cell,
+ *            coll_index [N_colloids], coll_cell_start, coll_cell_end
+ *            [Ntotal_cells_colloids] device outputs (an empty cell -1 / -2);
+ *            all ignored (may be NULL) when N_colloids < 2
+ *   status   NULL or a device int32 that receives SPH_OK, or
+ *            SPH_ERR_OUT_OF_GRID when a particle or colloid centre lay
+ *            outside its grid (it is filed under the nearest cell)
+ *   stream   the hipStream_t to queue on (NULL: the null stream).  The call
+ *            only queues: outputs are complete once the stream has run it.
+ * Calls that overlap on one GPU (from several threads or streams) share that
+ * GPU's scratch array and race: serialise them.  Each call clears what it
+ * needs cleared, so consecutive calls of different sizes are independent. */
+#define SPH_NOALLOC_MAX_PARTICLES 1100000
+#define SPH_NOALLOC_MAX_CELLS 65536
+#define SPH_NOALLOC_MAX_COLLOIDS 1000
+int sph_neighbours_search_noalloc(const sph_params_t *p, const double *x, const double *y,
+                                  const double *z, int32_t *particle_cell,
+                                  int32_t *particle_index, int32_t *cell_start,
+                                  int32_t *cell_end, const double *coll_x, const double *coll_y,
+                                  const double *coll_z, int32_t *coll_cell, int32_t *coll_index,
+                                  int32_t *coll_cell_start, int32_t *coll_cell_end,
+                                  int32_t *status, void *stream);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libsph_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["sph_device.hip", "sph_mlp.hip"]
+HIP_SOURCES = ["sph_device.hip", "sph_mlp.hip", "sph_noalloc.hip"]
 CXX_SOURCES = ["sph_host.cpp", "sph_capi.cpp"]
 
 

@@ -22,7 +22,7 @@ FP64, FP32 = 0, 1
 STATUS = {
     0: "SPH_OK", 1: "SPH_ERR_ARG", 2: "SPH_ERR_IO", 3: "SPH_ERR_PARSE",
     4: "SPH_ERR_GEOMETRY", 5: "SPH_ERR_DEVICE", 6: "SPH_ERR_OUT_OF_GRID",
-    7: "SPH_ERR_NAN", 8: "SPH_ERR_STATE",
+    7: "SPH_ERR_NAN", 8: "SPH_ERR_STATE", 9: "SPH_ERR_CAPACITY",
 }
 
 # The C-ABI entry points (include/sph.h).  Tests check that each is exported.
@@ -40,7 +40,7 @@ ENTRY_POINTS = [
     "sph_batch_size", "sph_batch_prepare", "sph_batch_step", "sph_batch_synchronize",
     "sph_batch_current_step", "sph_batch_get_field", "sph_batch_kinetic_energy",
     "sph_batch_observables", "sph_batch_set_launch", "sph_batch_get_launch",
-    "sph_batch_autotune", "sph_batch_failed",
+    "sph_batch_autotune", "sph_batch_failed", "sph_neighbours_search_noalloc",
 ]
 # include/sph_surrogate.h (bound in surrogate.py)
 SURROGATE_ENTRY_POINTS = [
@@ -190,6 +190,8 @@ def library(path=None):
     lib.sph_batch_get_launch.argtypes = [vp, vp, vp]
     lib.sph_batch_autotune.restype = i32
     lib.sph_batch_autotune.argtypes = [vp, i32, vp, vp, vp]
+    lib.sph_neighbours_search_noalloc.restype = i32
+    lib.sph_neighbours_search_noalloc.argtypes = [ctypes.POINTER(SphParams)] + [vp] * 16
     if path is None:
         _LIB = lib
     return lib
