@@ -37,7 +37,20 @@ import surrogate  # noqa: E402
 import sweep  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 vector peak: 157.3 TFLOP/s counts packed FP32 (v_pk_fma_f32, two
+# lanes per instruction); this build compiles without packed FP32, whose
+# ceiling is half of it.
 FP32_VALU_PEAK_TFLOPS = 157.3
+FP32_VALU_UNPACKED_TFLOPS = 78.65
+# Compulsory HBM bytes per particle-step of the whole FP32-form 3D step
+# (hash 32 + sort 16 + reorder 136 + pass A 92 + pass B 156).
+STEP_BYTES_3D_FP32 = 432
+# Flops of one fluid-fluid pair in k_pairs from the reference's formulas
+# (src/kernel_vel_gradients.cu:632-640, src/kernel_forces.cu:429-440), an FMA
+# counted as two: separation and r^2 8, r and 1/r 2, e 3, gradW 16, v_ij 3,
+# velocity gradient 22, reversible stress force 30, e.v_ij 5, viscous
+# coefficients 6, viscous force 12.
+FLOPS_PER_PAIR = 107
 
 
 def kernel_bytes(phase, dim, rbytes, store_gradv=False):
@@ -94,16 +107,25 @@ def make_system(precision, n, n_colloids, tmpdir, seed=3):
     return s, cfg
 
 
-def time_gpu(precision, n, n_colloids, steps, warmup, tmpdir):
+def time_gpu(precision, n, n_colloids, steps, warmup, tmpdir, amortize_steps=300):
     s, cfg = make_system(precision, n, n_colloids, tmpdir)
     s.prepare()
     s.step(warmup)
     s.synchronize()
+    c0 = s.counters()
     t0 = time.perf_counter()
     s.step(steps)
     s.synchronize()
     t1 = time.perf_counter()
+    c1 = s.counters()
     elapsed = t1 - t0
+    # A window of several Verlet-list rebuild periods: the amortized cost of a
+    # step, rebuilds included (the timed window above may hold none).
+    t2 = time.perf_counter()
+    s.step(amortize_steps)
+    s.synchronize()
+    t3 = time.perf_counter()
+    c2 = s.counters()
     # Per-phase device time with HIP events on the library's own stream.
     s.set_timing(True)
     s.get_timing()  # reset
@@ -111,25 +133,68 @@ def time_gpu(precision, n, n_colloids, steps, warmup, tmpdir):
     s.synchronize()
     phases = s.get_timing()
     s.set_timing(False)
+    counts = s.counters()
     ke = s.kinetic_energy()
     N = s.N
     s.close()
-    return dict(N=N, elapsed=elapsed, phases=phases, ke=ke, cfg=cfg)
+    return dict(N=N, elapsed=elapsed, phases=phases, ke=ke, cfg=cfg,
+                rebuilds_in_window=c1["rebuilds"] - c0["rebuilds"],
+                amortized=dict(steps=amortize_steps, seconds=t3 - t2,
+                               rebuilds=c2["rebuilds"] - c1["rebuilds"]),
+                pairs=counts["pairs"], list_entries=counts["list_entries"])
 
 
-def cpu_baseline(n, n_colloids, steps, tmpdir):
+def _time_oracle(o, warmup, steps, repeats):
+    """Median wall time of `steps` oracle steps over `repeats` repeats after
+    `warmup` untimed steps."""
+    o.prepare()
+    o.step(warmup)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o.step(steps)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times
+
+
+def cpu_baseline(n, n_colloids, tmpdir, steps=5, warmup=10, repeats=3, with_small=True):
+    """The FP64 OpenMP oracle on this host's cores: the 1M-particle workload
+    (the bench's), and the 2D Couette (full 2000 steps) and 3D 64^3 periodic
+    flows (20 steps), each after `warmup` steps, median of `repeats`; the
+    AVX-512 build on hosts that have it."""
     import oracle
 
+    lib = oracle.LIB_V4_PATH if oracle.host_has_avx512() and os.path.exists(oracle.LIB_V4_PATH) \
+        else oracle.LIB_PATH
+    out = {}
     s, _ = make_system(sph_system.FP64, n, n_colloids, tmpdir)
-    o = oracle.Oracle.from_system(s)
+    o = oracle.Oracle.from_system(s, lib_path=lib)
     N = s.N
     s.close()
-    o.prepare()
-    t0 = time.perf_counter()
-    o.step(steps)
-    t1 = time.perf_counter()
-    return dict(value=N * steps / (t1 - t0), cores=oracle.num_threads(), N=N, steps=steps,
-                seconds=t1 - t0)
+    med, times = _time_oracle(o, warmup, steps, repeats)
+    del o
+    out["main"] = dict(value=N * steps / med, N=N, steps=steps, seconds=times)
+    if with_small:
+        for key, cfg, nsteps in (("couette_2d_64x64", inputs.couette_2d(n=64), 2000),
+                                 ("periodic_3d_64cubed", inputs.sine_3d(n=64), 20)):
+            path = os.path.join(tmpdir, f"input_cpu_{key}")
+            inputs.write_input(path, cfg)
+            q = sph_system.System(sph_system.FP64)
+            q.read_input(path)
+            q.initialize()
+            inputs.seeded_velocity(q, 1e-3, 1)
+            o = oracle.Oracle.from_system(q, lib_path=lib)
+            Nq = q.N
+            q.close()
+            med, times = _time_oracle(o, warmup, nsteps, repeats)
+            del o
+            out[key] = dict(value=Nq * nsteps / med, N=Nq, steps=nsteps, seconds=times)
+    out["cores"] = oracle.num_threads(lib)
+    out["cpu"] = oracle.cpu_model()
+    out["nproc"] = os.cpu_count()
+    out["isa"] = "x86-64-v4 (AVX-512)" if lib == oracle.LIB_V4_PATH else "x86-64-v3 (AVX2)"
+    out["warmup"], out["repeats"] = warmup, repeats
+    return out
 
 
 def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
@@ -222,7 +287,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=100, help="lattice sites per axis")
     ap.add_argument("--colloids", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -259,9 +324,9 @@ def main():
                   f"error {pipe['wall_stress_rel_err_median']:.3%}", file=sys.stderr, flush=True)
         cpu = None
         if not a.no_cpu:
-            cpu = cpu_baseline(a.n, a.colloids, a.cpu_steps, tmpdir)
-            print(f"# cpu oracle: {cpu['steps']} steps in {cpu['seconds']:.1f}s on "
-                  f"{cpu['cores']} threads", file=sys.stderr, flush=True)
+            cpu = cpu_baseline(a.n, a.colloids, tmpdir, steps=a.cpu_steps)
+            print(f"# cpu oracle ({cpu['isa']}, {cpu['cores']} threads, {cpu['cpu']}): "
+                  f"{cpu['main']['value']:.3e} particle-steps/s", file=sys.stderr, flush=True)
 
     N = r32["N"]
     value = N * a.steps / r32["elapsed"]
@@ -275,6 +340,11 @@ def main():
              "vv1": "k_vv1", "vv2": "k_vv2", "walls_colloids": "k_walls"}.get(dom, dom)
     traffic = measured_traffic(kname, N)
     step_ms = 1e3 * r32["elapsed"] / a.steps
+    am = r32["amortized"]
+    amortized_value = N * am["steps"] / am["seconds"]
+    # flop ceiling of the pair pass: evaluated pairs = neighbours within rcut
+    pair_tflops = FLOPS_PER_PAIR * r32["pairs"] / (ph["forces"][0] / ph["forces"][1] * 1e-3) / 1e12 \
+        if ph.get("forces", (0, 0))[1] else None
     line = {
         "metric": "particle-steps/s, 3D 1M-particle Oldroyd-B (FP32 vs FP64); % HBM peak; sweep speedup",
         "value": value,
@@ -302,6 +372,14 @@ def main():
             "ms_per_step": 1e3 * r64["elapsed"] / a.steps,
         },
         "phases_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in ph.items() if v[1]},
+        # Verlet lists are rebuilt when a particle has moved half the skin:
+        # rebuilds inside the timed window, and a longer window's amortized rate
+        "rebuilds_in_window": r32["rebuilds_in_window"],
+        "amortized": {"steps": am["steps"], "rebuilds": am["rebuilds"],
+                      "ms_per_step": 1e3 * am["seconds"] / am["steps"],
+                      "value": amortized_value},
+        # the whole step's compulsory bytes (432 B per particle-step) at `value`
+        "step_hbm_frac": STEP_BYTES_3D_FP32 * value / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "kernel": kname,
             "bound": "hbm",
@@ -318,15 +396,32 @@ def main():
             "valu_busy_profiled": VALU_BUSY.get(kname),
             "valu_busy_source": PAIR_PROFILE if kname in VALU_BUSY else None,
         },
+        # what bounds the pair pass instead: FP32 vector arithmetic
+        "valu_roofline": None if pair_tflops is None else {
+            "kernel": "k_pairs", "bound": "valu",
+            "achieved": pair_tflops, "unit": "TFLOP/s",
+            "peak": FP32_VALU_PEAK_TFLOPS, "frac": pair_tflops / FP32_VALU_PEAK_TFLOPS,
+            "peak_unpacked": FP32_VALU_UNPACKED_TFLOPS,
+            "frac_unpacked": pair_tflops / FP32_VALU_UNPACKED_TFLOPS,
+            "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": r32["pairs"],
+            "list_entries_per_launch": r32["list_entries"],
+            "note": "peak counts packed FP32 (v_pk_fma_f32); this build is compiled without "
+                    "packed FP32, whose ceiling is peak_unpacked",
+        },
         "sweep": sw,
         "sweep_pipeline": pipe,
         "cpu_baseline": None if cpu is None else {
-            "value": cpu["value"],
+            "value": cpu["main"]["value"],
             "unit": "particle-steps/s",
             "cores": cpu["cores"],
             "kind": "port",
-            "sample": f"FP64 OpenMP oracle, same workload (N={cpu['N']}), "
-                      f"{cpu['steps']} steps after the initial force pass",
+            "sample": f"FP64 OpenMP oracle ({cpu['isa']}, {cpu['cpu']}, nproc "
+                      f"{cpu['nproc']}), same workload (N={cpu['main']['N']}): "
+                      f"{cpu['main']['steps']} steps after {cpu['warmup']} warm-up steps, "
+                      f"median of {cpu['repeats']}",
+            "other_configs": {k: {"value": v["value"], "N": v["N"], "steps": v["steps"]}
+                              for k, v in cpu.items()
+                              if isinstance(v, dict) and k != "main"},
         },
     }
     print(json.dumps(line), flush=True)

@@ -202,6 +202,12 @@ int sph_write_restart(sph_handle h, const char *path);
  * last reset and `calls` the launch counts. */
 int sph_set_timing(sph_handle h, int enable);
 int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]);
+/* Work counters (no reference counterpart; they size the work of a step for
+ * the roofline): out[0] steps run, out[1] Verlet-list rebuilds since the first
+ * device call, out[2] neighbour pairs within rcut found by the last density
+ * pass (sum over particles of their neighbour counts), out[3] entries of the
+ * Verlet lists of the last rebuild (0 when the step runs without them). */
+int sph_get_counters(sph_handle h, int64_t out[4]);
 
 /* ---- batches of independent simulations --------------------------------- */
 /* A batch runs many independent simulations of uneven size - a parameter

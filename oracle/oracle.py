@@ -12,8 +12,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_V4_PATH = os.path.join(HERE, "liboracle_v4.so")   # AVX-512 build (CPU baseline)
 
 _LIB = None
+_LIBS = {}
 
 
 def build_oracle():
@@ -21,15 +23,36 @@ def build_oracle():
     return LIB_PATH
 
 
-def library():
+def host_has_avx512():
+    """True when this host's CPU has the AVX-512 subsets x86-64-v4 requires."""
+    try:
+        flags = open("/proc/cpuinfo").read().split("flags", 2)[1].split("\n", 1)[0].split()
+    except (OSError, IndexError):
+        return False
+    return all(f in flags for f in ("avx512f", "avx512dq", "avx512cd", "avx512bw", "avx512vl"))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def library(path=None):
+    """The oracle library (default: the x86-64-v3 build the tests use)."""
     global _LIB
-    if _LIB is not None:
-        return _LIB
-    if not os.path.exists(LIB_PATH):
+    path = path or LIB_PATH
+    if path in _LIBS:
+        return _LIBS[path]
+    if not os.path.exists(path):
         build_oracle()
     from sph_system import SphParams  # the shared parameter-struct layout
 
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     lib.orc_create.restype = vp
     lib.orc_create.argtypes = [ctypes.POINTER(SphParams)]
@@ -66,7 +89,9 @@ def library():
     lib.orc_kernel_gradW.argtypes = [ctypes.POINTER(SphParams), ctypes.c_double]
     lib.orc_density_brute_force.restype = i32
     lib.orc_density_brute_force.argtypes = [vp, vp]
-    _LIB = lib
+    _LIBS[path] = lib
+    if path == LIB_PATH:
+        _LIB = lib
     return lib
 
 
@@ -223,17 +248,17 @@ DERIVED_FIELDS = (
 
 
 class Oracle:
-    def __init__(self, params):
-        self.lib = library()
+    def __init__(self, params, lib_path=None):
+        self.lib = library(lib_path)
         self.params = params
         self.h = self.lib.orc_create(ctypes.byref(params))
 
     @classmethod
-    def from_system(cls, system, slip=None):
+    def from_system(cls, system, slip=None, lib_path=None):
         """Oracle with the current state of a sph_system.System.  Its
         parameters are the system's raw input values with every derived
         constant recomputed here (derive_params), not copied."""
-        o = cls(derive_params(system.params, slip=slip))
+        o = cls(derive_params(system.params, slip=slip), lib_path=lib_path)
         for n in STATE_FIELDS + SETUP_FIELDS:
             o.set(n, system.get(n))
         if system.params.N_colloids:
@@ -375,5 +400,5 @@ def kernel_gradW(params, r):
     return library().orc_kernel_gradW(ctypes.byref(params), float(r))
 
 
-def num_threads():
-    return library().orc_num_threads()
+def num_threads(lib_path=None):
+    return library(lib_path).orc_num_threads()

@@ -369,6 +369,20 @@ int sph_set_timing(sph_handle h, int enable) {
   return SPH_OK;
 }
 
+int sph_get_counters(sph_handle h, int64_t out[4]) {
+  if (!h || !out) return SPH_ERR_ARG;
+  out[0] = h->step;
+  out[1] = out[2] = out[3] = 0;
+  if (!h->dev) return SPH_OK;
+  int64_t c[3];
+  int rc = h->dev->counters(0, c, h->err);
+  if (rc) return rc;
+  out[1] = c[0];
+  out[2] = c[1];
+  out[3] = c[2];
+  return SPH_OK;
+}
+
 int sph_get_timing(sph_handle h, double ms[8], int64_t calls[8]) {
   if (!h || !ms || !calls) return SPH_ERR_ARG;
   if (!h->dev) {

@@ -76,6 +76,11 @@ class Device {
 
   virtual void set_timing(bool on) = 0;
   virtual int get_timing(double ms[8], int64_t calls[8], std::string &err) = 0;
+  // Work counters of simulation `sim`: [0] Verlet-list rebuilds since the
+  // upload, [1] neighbour pairs within rcut of the last density pass (sum over
+  // particles of their neighbour counts), [2] Verlet-list entries of the last
+  // build (0 without Verlet lists).
+  virtual int counters(int sim, int64_t out[3], std::string &err) = 0;
 };
 
 // precision: SPH_FP64 or SPH_FP32

@@ -156,6 +156,7 @@ struct VerletCtl {
   // step's copy-back clears it for the next step (after which an overflowing
   // build raises it again).
   int *any;
+  unsigned long long *rebuilds;  // [nsim] list builds so far (instrumentation)
 };
 
 struct VList {
@@ -1449,6 +1450,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const int b = tile_of(T), sim = T.blk_sim[b];
   if (!V.rebuild[sim]) return;
   const int i = T.blk_first[b] + threadIdx.x;
+  if (i == T.pbeg[sim]) atomicAdd(&V.rebuilds[sim], 1ull);
   if (i >= T.pbeg[sim + 1]) return;
   const sph_params_t &P = T.P[sim];
   const int *__restrict__ cs = csg + T.cell_off[sim];
@@ -2551,6 +2553,28 @@ __global__ __launch_bounds__(kBlock) void k_observables_final(SimTable T, Bodies
     for (int k = 0; k < 6; ++k) out[6 * sim + k] = acc[k];
 }
 
+// Sums of the neighbour counts of the last density pass and of the Verlet
+// list lengths over particles [p0, p1) (instrumentation: the pair work of a
+// step).
+__global__ __launch_bounds__(kBlock) void k_count_pairs(const int *__restrict__ nl_count,
+                                                        const int *__restrict__ vl_count, int p0,
+                                                        int p1, unsigned long long *out) {
+  unsigned long long a = 0, v = 0;
+  for (int i = p0 + blockIdx.x * blockDim.x + threadIdx.x; i < p1; i += gridDim.x * blockDim.x) {
+    a += (unsigned)nl_count[i];
+    if (vl_count) v += (unsigned)vl_count[i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    v += __shfl_down(v, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], a);
+    atomicAdd(&out[1], v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // owner of the device state
 // ---------------------------------------------------------------------------
@@ -2725,6 +2749,9 @@ class DeviceImpl final : public Device {
       HIPCHK(any_.alloc(1));
       HIPCHK(hipMemset(any_.p, 0, sizeof(int)));
     }
+    HIPCHK(rebuilds_.alloc(S));
+    HIPCHK(hipMemset(rebuilds_.p, 0, S * sizeof(unsigned long long)));
+    HIPCHK(count_out_.alloc(2));
     verlet_valid_ = verlet_dirty_ = in_verlet_ = false;
     HIPCHK(nl_count_.alloc(N_));
     HIPCHK(nl_queue_.alloc(N_));
@@ -3461,6 +3488,22 @@ class DeviceImpl final : public Device {
     return SPH_OK;
   }
 
+  int counters(int sim, int64_t out[3], std::string &err) override {
+    const SimInfo &si = sims_[sim];
+    unsigned long long r = 0, sums[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&r, rebuilds_.p + sim, sizeof(r), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemsetAsync(count_out_.p, 0, 2 * sizeof(unsigned long long), stream_));
+    k_count_pairs<<<std::min(1024, blocks(si.n)), kBlock, 0, stream_>>>(
+        nl_count_.p, verlet_ok_ ? vl_count_.p : nullptr, si.pbeg, si.pbeg + si.n, count_out_.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sums, count_out_.p, sizeof(sums), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    out[0] = (int64_t)r;
+    out[1] = (int64_t)sums[0];
+    out[2] = (int64_t)sums[1];
+    return SPH_OK;
+  }
+
   void set_timing(bool on) override { timing_ = on; }
 
   int get_timing(double ms[8], int64_t calls[8], std::string &err) override {
@@ -3579,7 +3622,7 @@ class DeviceImpl final : public Device {
   }
   VerletCtl vctl() {
     VerletCtl v{vrebuild_.p, voverflow_.p, ref4_.p, 0.49f * skin_frac_, in_verlet_ ? 1 : 0,
-                any_.p};
+                any_.p, rebuilds_.p};
     return v;
   }
 
@@ -3656,6 +3699,8 @@ class DeviceImpl final : public Device {
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
   DevBuf<int> any_;  // [1] batch rebuild flag (VerletCtl::any)
+  DevBuf<unsigned long long> rebuilds_;   // [nsim] Verlet-list builds (VerletCtl::rebuilds)
+  DevBuf<unsigned long long> count_out_;  // [2] k_count_pairs sums
   DevBuf<float4> ref4_;
   bool have_cells_ = false, id_dirty_ = false, timing_ = false;
   hipStream_t stream_ = nullptr;

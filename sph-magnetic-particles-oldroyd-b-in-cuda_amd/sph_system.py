@@ -35,7 +35,8 @@ ENTRY_POINTS = [
     "sph_calculate_force_colloids", "sph_calculate_forces", "sph_V_Verlet_step1",
     "sph_V_Verlet_step2", "sph_set_step", "sph_prepare", "sph_step", "sph_synchronize",
     "sph_current_step", "sph_run", "sph_kinetic_energy", "sph_print_output",
-    "sph_write_restart", "sph_set_timing", "sph_get_timing", "sph_observables",
+    "sph_write_restart", "sph_set_timing", "sph_get_timing", "sph_get_counters",
+    "sph_observables",
     "sph_batch_create", "sph_batch_destroy", "sph_batch_last_error", "sph_batch_add",
     "sph_batch_size", "sph_batch_prepare", "sph_batch_step", "sph_batch_synchronize",
     "sph_batch_current_step", "sph_batch_get_field", "sph_batch_kinetic_energy",
@@ -165,6 +166,8 @@ def library(path=None):
     lib.sph_observables.argtypes = [vp, vp]
     lib.sph_get_timing.restype = i32
     lib.sph_get_timing.argtypes = [vp, vp, vp]
+    lib.sph_get_counters.restype = i32
+    lib.sph_get_counters.argtypes = [vp, vp]
     lib.sph_batch_create.restype = vp
     lib.sph_batch_create.argtypes = [i32]
     lib.sph_batch_destroy.restype = None
@@ -379,6 +382,14 @@ class System:
         calls = np.zeros(8, np.int64)
         self._check(self.lib.sph_get_timing(self.h, _ptr(ms), _ptr(calls)), "get_timing")
         return {PHASES[k]: (float(ms[k]), int(calls[k])) for k in range(8)}
+
+    def counters(self):
+        """sph_get_counters: steps run, Verlet rebuilds, neighbour pairs within
+        rcut of the last density pass, Verlet-list entries of the last build."""
+        out = np.zeros(4, np.int64)
+        self._check(self.lib.sph_get_counters(self.h, _ptr(out)), "get_counters")
+        return dict(steps=int(out[0]), rebuilds=int(out[1]), pairs=int(out[2]),
+                    list_entries=int(out[3]))
 
 
 class Batch:
