@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SPH_ORACLE_LIB selects another build (e.g. liboracle_asan.so, make asan).
+LIB_PATH = os.environ.get("SPH_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 LIB_V4_PATH = os.path.join(HERE, "liboracle_v4.so")   # AVX-512 build (CPU baseline)
 
 _LIB = None

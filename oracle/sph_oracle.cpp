@@ -316,6 +316,7 @@ static int field_ptr(orc_system *s, const char *name, double **dp, int32_t **ip,
 int orc_set_field(orc_system *s, const char *name, const void *src, int64_t count) {
   double *dp; int32_t *ip; int64_t n;
   if (field_ptr(s, name, &dp, &ip, &n) || n != count) return SPH_ERR_ARG;
+  if (n == 0) return SPH_OK;  // (an empty field's storage may be null)
   if (dp) std::memcpy(dp, src, n * sizeof(double));
   else std::memcpy(ip, src, n * sizeof(int32_t));
   return SPH_OK;
@@ -324,6 +325,7 @@ int orc_set_field(orc_system *s, const char *name, const void *src, int64_t coun
 int orc_get_field(orc_system *s, const char *name, void *dst, int64_t count) {
   double *dp; int32_t *ip; int64_t n;
   if (field_ptr(s, name, &dp, &ip, &n) || n != count) return SPH_ERR_ARG;
+  if (n == 0) return SPH_OK;
   if (dp) std::memcpy(dst, dp, n * sizeof(double));
   else std::memcpy(dst, ip, n * sizeof(int32_t));
   return SPH_OK;
@@ -331,20 +333,20 @@ int orc_get_field(orc_system *s, const char *name, void *dst, int64_t count) {
 
 int orc_get_cell_list(orc_system *s, int32_t *pc, int32_t *pi, int32_t *cs, int32_t *ce) {
   const int N = s->p.N, NC = s->p.Ntotal_cells;
-  if (pc) std::memcpy(pc, s->particle_cell.data(), N * sizeof(int32_t));
-  if (pi) std::memcpy(pi, s->particle_index.data(), N * sizeof(int32_t));
-  if (cs) std::memcpy(cs, s->cell_start.data(), NC * sizeof(int32_t));
-  if (ce) std::memcpy(ce, s->cell_end.data(), NC * sizeof(int32_t));
+  if (pc && N) std::memcpy(pc, s->particle_cell.data(), N * sizeof(int32_t));
+  if (pi && N) std::memcpy(pi, s->particle_index.data(), N * sizeof(int32_t));
+  if (cs && NC) std::memcpy(cs, s->cell_start.data(), NC * sizeof(int32_t));
+  if (ce && NC) std::memcpy(ce, s->cell_end.data(), NC * sizeof(int32_t));
   return SPH_OK;
 }
 
 int orc_get_colloid_cell_list(orc_system *s, int32_t *cc, int32_t *ci, int32_t *cs,
                               int32_t *ce) {
   const int Nc = s->p.N_colloids, NCC = std::max(s->p.Ntotal_cells_colloids, 0);
-  if (cc) std::memcpy(cc, s->coll_cell.data(), Nc * sizeof(int32_t));
-  if (ci) std::memcpy(ci, s->coll_index.data(), Nc * sizeof(int32_t));
-  if (cs) std::memcpy(cs, s->coll_cell_start.data(), NCC * sizeof(int32_t));
-  if (ce) std::memcpy(ce, s->coll_cell_end.data(), NCC * sizeof(int32_t));
+  if (cc && Nc) std::memcpy(cc, s->coll_cell.data(), Nc * sizeof(int32_t));
+  if (ci && Nc) std::memcpy(ci, s->coll_index.data(), Nc * sizeof(int32_t));
+  if (cs && NCC) std::memcpy(cs, s->coll_cell_start.data(), NCC * sizeof(int32_t));
+  if (ce && NCC) std::memcpy(ce, s->coll_cell_end.data(), NCC * sizeof(int32_t));
   return SPH_OK;
 }
 

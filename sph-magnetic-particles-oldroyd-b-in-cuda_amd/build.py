@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsph_amd.so")
+LIB_ASAN = os.path.join(HERE, "libsph_amd_asan.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -31,7 +32,11 @@ def _run(cmd):
     return res.stdout
 
 
-def build(arch="gfx950", jobs=4, verbose=False):
+def build(arch="gfx950", jobs=4, verbose=False, sanitize=False):
+    """Builds libsph_amd.so; with `sanitize`, libsph_amd_asan.so instead, whose
+    host C++ (input, set-up, writers, C-ABI) is compiled by g++ with
+    AddressSanitizer and UndefinedBehaviorSanitizer (the device objects are the
+    normal ones)."""
     os.makedirs(BUILD, exist_ok=True)
     hip_flags = [
         "-O3", "-std=c++17", "-fPIC", f"--offload-arch={arch}",
@@ -39,6 +44,9 @@ def build(arch="gfx950", jobs=4, verbose=False):
         "-munsafe-fp-atomics", "-Wno-unused-command-line-argument",
     ]
     cxx_flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+    if sanitize:
+        cxx_flags = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                     "-fno-sanitize-recover=undefined"] + cxx_flags[1:]
     cmds = []
     objs = []
     for src in HIP_SOURCES:
@@ -46,7 +54,7 @@ def build(arch="gfx950", jobs=4, verbose=False):
         cmds.append([HIPCC] + hip_flags + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
     for src in CXX_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(BUILD, src + (".asan.o" if sanitize else ".o"))
         cmds.append(["g++"] + cxx_flags + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -54,10 +62,14 @@ def build(arch="gfx950", jobs=4, verbose=False):
     if verbose:
         for o in outs:
             sys.stdout.write(o)
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={arch}", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    lib = LIB_ASAN if sanitize else LIB
+    tmp = lib + ".tmp"
+    link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={arch}", "-o", tmp] + objs
+    if sanitize:
+        link += ["-Xarch_host", "-fsanitize=address,undefined"]
+    _run(link)
+    os.replace(tmp, lib)
+    return lib
 
 
 def clean():
@@ -72,7 +84,9 @@ if __name__ == "__main__":
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("--clean", action="store_true")
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--sanitize", action="store_true",
+                    help="build libsph_amd_asan.so (host C++ under ASan + UBSan)")
     a = ap.parse_args()
     if a.clean:
         clean()
-    print(build(a.arch, a.jobs, a.verbose))
+    print(build(a.arch, a.jobs, a.verbose, a.sanitize))

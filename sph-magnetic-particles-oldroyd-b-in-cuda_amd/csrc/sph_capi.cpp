@@ -185,6 +185,7 @@ int sph_get_field(sph_handle h, const char *name, void *dst, int64_t count) {
     if (rc) return rc;
     find_field(h->hs, name, f);  // storage may have been reallocated
   }
+  if (count == 0) return SPH_OK;  // (an empty field's storage may be null)
   if (f.d) std::memcpy(dst, f.d, count * sizeof(double));
   else std::memcpy(dst, f.i, count * sizeof(int32_t));
   return SPH_OK;
@@ -209,8 +210,10 @@ int sph_set_field(sph_handle h, const char *name, const void *src, int64_t count
     if (rc) return rc;
     find_field(h->hs, name, f);
   }
-  if (f.d) std::memcpy(f.d, src, count * sizeof(double));
-  else std::memcpy(f.i, src, count * sizeof(int32_t));
+  if (count > 0) {  // (an empty field's storage may be null)
+    if (f.d) std::memcpy(f.d, src, count * sizeof(double));
+    else std::memcpy(f.i, src, count * sizeof(int32_t));
+  }
   if (h->dev && f.device) return h->dev->upload_field(0, h->hs, name, h->err);
   return SPH_OK;
 }
@@ -528,6 +531,7 @@ int sph_batch_get_field(sph_batch_handle b, int sim, const char *name, void *dst
     if (rc) return rc;
     find_field(hs, name, f);
   }
+  if (count == 0) return SPH_OK;  // (an empty field's storage may be null)
   if (f.d) std::memcpy(dst, f.d, count * sizeof(double));
   else std::memcpy(dst, f.i, count * sizeof(int32_t));
   return SPH_OK;

@@ -16,7 +16,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsph_amd.so")
+# SPH_LIB selects another build of the library (e.g. the sanitizer build of
+# build.py --sanitize).
+LIB_PATH = os.environ.get("SPH_LIB") or os.path.join(HERE, "libsph_amd.so")
 
 FP64, FP32 = 0, 1
 STATUS = {
