@@ -242,33 +242,38 @@ def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
     }
 
 
-def time_pipeline(n_train, n_hold, n_sweep, steps, n=48, seed=21, fit_steps=6000, log=None):
+def time_pipeline(n_train, n_hold, n_sweep, steps, n_range=(48, 48), seed=21, fit_steps=6000,
+                  lr=1e-3, log=None):
     """The learned sweep: SPH runs train the MLP ensemble; a new sweep is then
     answered by the ensemble, with the points whose spread of the wall stress
     exceeds 5 % re-run by batched SPH.  Compared with running the whole new
     sweep with batched SPH: wall time and wall-stress error.  The lattice is
     fixed (n x n) so that the outputs are functions of the four inputs; the
     physical parameters are drawn as in the uneven-size sweep."""
-    train = sweep.resolve(sweep.sample_points(n_train + n_hold, seed=seed, n_range=(n, n)))
+    train = sweep.resolve(sweep.sample_points(n_train + n_hold, seed=seed, n_range=n_range))
     Y, tinfo = sweep.run_points(train, steps, log=log)
     X = surrogate.feature_matrix(train)
     model = surrogate.Surrogate(members=5, seed=seed)
     t0 = time.perf_counter()
-    model.fit(X[:n_train], Y[:n_train], steps=fit_steps, batch=256, lr=2e-3, lr_end=1e-5)
+    model.fit(X[:n_train], Y[:n_train], steps=fit_steps, batch=256, lr=lr, lr_end=1e-5)
     t_fit = time.perf_counter() - t0
     mh, _ = model.predict(X[n_train:])
     ok = np.isfinite(Y[n_train:, 0])
     held_err = np.abs(mh[ok, 0] - Y[n_train:][ok, 0]) / np.abs(Y[n_train:][ok, 0])
-    pts = sweep.resolve(sweep.sample_points(n_sweep, seed=seed + 1, n_range=(n, n)))
+    pts = sweep.resolve(sweep.sample_points(n_sweep, seed=seed + 1, n_range=n_range))
     t0 = time.perf_counter()
     Yref, pinfo = sweep.run_points([dict(p) for p in pts], steps, log=log)
     t_plain = time.perf_counter() - t0
     Ys, info = surrogate.sweep_with_fallback(pts, model, steps, threshold=0.05, log=log)
     ok = np.isfinite(Yref[:, 0]) & np.isfinite(Ys[:, 0])
     err = np.abs(Ys[ok, 0] - Yref[ok, 0]) / np.abs(Yref[ok, 0])
+    lattice = (f"{n_range[0]}x{n_range[0]} lattice" if n_range[0] == n_range[1] else
+               f"n per side {n_range[0]}-{n_range[1]}")
+    settle = np.concatenate([tinfo["settle"], pinfo["settle"]])
     return {
-        "workload": f"{n_sweep}-point 2D sweep ({n}x{n} lattice, {steps} steps, dt {sweep.DT}); "
+        "workload": f"{n_sweep}-point 2D sweep ({lattice}, {steps} steps, dt {sweep.DT}); "
                     f"surrogate trained on {n_train} SPH runs, {n_hold} held out",
+        "wall_stress_settle_median": float(np.nanmedian(settle)),
         "plain_batched_s": t_plain, "surrogate_s": info["seconds"],
         "speedup": t_plain / info["seconds"],
         "fallback_points": info["fallback"], "inference_s": info["inference_s"],

@@ -160,7 +160,9 @@ struct VerletCtl {
 };
 
 struct VList {
-  int4 *idx;   // (cap / 4) * N groups, column-major like NList
+  // (cap / 4) * N groups, column-major like NList; an entry is the neighbour's
+  // index within its simulation (sorted position - pbeg)
+  int4 *idx;
   int *count;  // N: entries found at the build (may exceed cap)
   // Per step, the density pass marks which entries are inside rcut: bit e % 32
   // of word e / 32 (column-major, words * N), so the pair pass visits exactly
@@ -244,9 +246,23 @@ __device__ __forceinline__ Fix fix_of(const sph_params_t &P) {
 __device__ __forceinline__ unsigned fix_enc(double x, double s, double o) {
   return (unsigned)(long long)rint((x - o) * s);
 }
+// Loads of the neighbour lists' index groups and hit-mask words: each is read
+// once per pass, so they are marked non-temporal, which keeps the L2 for the
+// positions and pair records the entries point at (re-read by every particle
+// that lists the same neighbour).
+__device__ __forceinline__ unsigned ld_stream(const unsigned *p) {
+  return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ int4 ld_stream(const int4 *p) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p));
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+
+// (.w = 1 marks a stored position: a buffer load past the records reads 0)
 __device__ __forceinline__ uint4 fix_pos(const Fix &f, double x, double y, double z) {
   return make_uint4(fix_enc(x, f.s[0], f.o[0]), fix_enc(y, f.s[1], f.o[1]),
-                    fix_enc(z, f.s[2], f.o[2]), 0u);
+                    fix_enc(z, f.s[2], f.o[2]), 1u);
 }
 template <int DIM>
 __device__ __forceinline__ void fix_sep(const float u[3], uint4 a, uint4 b, float r[3]) {
@@ -365,6 +381,19 @@ __device__ __forceinline__ R sph_W(R r, R rcut_inv, R cw) {
   if (q3 < R(2)) t = t - b5;
   if (q3 < R(1)) t = t + c5;
   return cw * t;
+}
+
+// The spline polynomial of kernel_W without cw, from q3 = 3 r / rcut:
+// (a^5 - 6 b^5) + 15 c^5 with a = 3 - q3, b = 2 - q3, c = 1 - q3 (terms dropped
+// past their range), so that a sum of W is cw times a sum of these.
+template <class R>
+__device__ __forceinline__ R sph_W_poly(R q3) {
+  R a = R(3) - q3, a2 = a * a, t = a2 * a2 * a;
+  R b = R(2) - q3, b2 = b * b, b5 = R(6) * (b2 * b2 * b);
+  R c = R(1) - q3, c2 = c * c, c5 = R(15) * (c2 * c2 * c);
+  if (q3 < R(2)) t = t - b5;
+  if (q3 < R(1)) t = t + c5;
+  return t;
 }
 
 // kernel_gradW (src/kernel_gradW.cu:13-53): c_gradw*((a^4 - 6 b^4) + 15 c^4).
@@ -1169,6 +1198,16 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
   const R rcutsq = (R)P.rcutsq, rinv = (R)P.rcut_inv, cw = (R)P.cw;
+  const R rinv3 = R(3) * rinv;
+  // The density sum term of a hit at r^2.  FP32 form: the spline polynomial
+  // at q3 = r (3 / rcut), the sum scaled by cw once (d = cw (66 + sum); W(0) =
+  // 66 cw), in every path alike (Verlet walk and strip scan round the same
+  // way, so a particle's density does not depend on which path took it).
+  // FP64 form: kernel_W itself, summed from W(0) as the reference does.
+  auto w_term = [&](R r2) {
+    if constexpr (sizeof(R) == 4) return sph_W_poly<R>(pair_sqrt(r2) * rinv3);
+    else return sph_W<R>(pair_sqrt(r2), rinv, cw);
+  };
   const Box bx = box_of(P);
   const int kmax = L.kmax;
   int cnt = 0;
@@ -1176,64 +1215,69 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
   const bool vok = VL.idx && live && VL.count[i] <= VL.cap;
   if constexpr (sizeof(R) == 4) {
     if (VL.idx) {
-      // FP32 form, Verlet list: the hits in list order, W summed as they come.
-      // One 16-byte fixed-point position per entry; the lanes step through the
-      // list groups (four entries) together, so index loads are coalesced, and
-      // the positions of group g + 1 are loaded before group g is evaluated.  A
+      // FP32 form, Verlet list: the hits in list order.  One 16-byte
+      // fixed-point position per entry; the lanes step through the list groups
+      // (four entries) together, so index loads are coalesced and their
+      // addresses are uniform but for the lane offset.  A
       // lane past its list asks for an offset past the simulation's positions
-      // (zeros, no memory access).  The hits are marked in the mask words, stored
-      // as each fills.
+      // (zeros, no memory access).  The spline polynomial of the hits is summed
+      // and scaled by cw once (d = cw (66 + sum t), q = r (3 / rcut)); the hits
+      // are marked in the mask words, stored as each fills, and counted from them.
       const int vc = vok ? VL.count[i] : 0;
       const int4 *__restrict__ vl = VL.idx + i;
+      const int capg = VL.cap >> 2;  // groups per list: index loads stay inside it
       // (uniform by construction; readfirstlane keeps the descriptor scalar)
       const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
       const int pe = __builtin_amdgcn_readfirstlane(iend);
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<uint4 *>(S.pq + pb), 0, (int)((unsigned)(pe - pb) * 16u), 0x00020000);
       const unsigned oob = (unsigned)(pe - pb) * 16u;
-      const int lastg = max(vc - 1, 0) >> 2;
       CandSelf<R, DIM> me;
       me.init(S, i, P);
+      auto index_group = [&](int gi) { return ld_stream(&vl[(size_t)min(gi, capg - 1) * N]); };
       auto group_load = [&](const int4 &g4, int e0, uint4(&v)[4]) {
         const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           v[u] = __builtin_bit_cast(
               uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                         rs, (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 16u : oob), 0, 0));
+                         rs, (int)(e0 + u < vc ? (unsigned)jj[u] * 16u : oob), 0, 0));
       };
-      // q: indices of the current group, qn: of the following group
-      int4 q = vl[0];
-      int4 qn = vl[(size_t)min(1, lastg) * N];
-      uint4 cur[4];
-      group_load(q, 0, cur);
-      if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+      R tsum = R(0);
       unsigned mw = 0u;
+      auto group_eval = [&](int g, const uint4(&c)[4]) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const R r2 = (R)me.r2(Cand<R, DIM>{c[u]});
+          if (c[u].w != 0u && r2 < rcutsq) {  // (entries past the list read 0)
+            tsum = tsum + w_term(r2);
+            mw |= 1u << ((g + u) & 31);
+          }
+        }
+        if (g < vc && (((g + 4) & 31) == 0 || g + 4 >= vc)) {
+          VL.mask[(size_t)(g >> 5) * N + i] = mw;
+          cnt += __builtin_popcount(mw);
+          mw = 0u;
+        }
+      };
+      // The positions of the next group are in flight while a group is
+      // evaluated, and its indices one group further ahead.  (Each lane leaves
+      // the loop after its own list: finished lanes issue no loads; the group
+      // counter is the same in every lane still in it.)
+      int4 qn = index_group(1);
+      uint4 cur[4];
+      group_load(index_group(0), 0, cur);
 #pragma unroll 1
       for (int g = 0; g < vc; g += 4) {
         uint4 nxt[4];
         group_load(qn, g + 4, nxt);
-        q = qn;
-        qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (g + u < vc) {
-            const R r2 = (R)me.r2(Cand<R, DIM>{cur[u]});
-            if (r2 < rcutsq) {
-              d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
-              mw |= 1u << ((g + u) & 31);
-              cnt++;
-            }
-          }
-        }
-        if (((g + 4) & 31) == 0 || g + 4 >= vc) {
-          VL.mask[(size_t)(g >> 5) * N + i] = mw;
-          mw = 0u;
-        }
+        qn = index_group((g >> 2) + 2);
+        group_eval(g, cur);
 #pragma unroll
         for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
       }
       if (!live) return;
+      if (vok) d = cw * (R(66) + tsum);  // (W(0) = 66 cw: the particle itself)
     }
   } else {
     if (VL.idx) {
@@ -1265,20 +1309,21 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<double4 *>(S.pd + pb), 0, (int)((unsigned)(pe - pb) * 32u), 0x00020000);
       const unsigned oob = (unsigned)(pe - pb) * 32u;
+      // (a lane past its list re-reads its last group: no new lines)
       const int lastg = max(vc - 1, 0) >> 2;
       auto group_load = [&](const int4 &g4, int e0, u4(&v)[4][2]) {
         const int jj[4] = {g4.x, g4.y, g4.z, g4.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           // lane pa + k's position offset (broadcast in the pair) plus this lane's half
-          const int mine = (int)(e0 + u < vc ? (unsigned)(jj[u] - pb) * 32u : oob), ho = 16 * hb;
+          const int mine = (int)(e0 + u < vc ? (unsigned)jj[u] * 32u : oob), ho = 16 * hb;
           v[u][0] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xA0>(mine) + ho, 0, 0);
           v[u][1] = __builtin_amdgcn_raw_buffer_load_b128(rs, quad_pick<0xF5>(mine) + ho, 0, 0);
         }
       };
       // q: indices of the current group, qn: of the following group
-      int4 q = vl[0];
-      int4 qn = vl[(size_t)min(1, lastg) * N];
+      int4 q = ld_stream(&vl[0]);
+      int4 qn = ld_stream(&vl[(size_t)min(1, lastg) * N]);
       u4 cur[4][2];
       group_load(q, 0, cur);
       if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
@@ -1288,7 +1333,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
         u4 nxt[4][2];
         group_load(qn, g + 4, nxt);
         q = qn;
-        qn = vl[(size_t)min((g + 8) >> 2, lastg) * N];
+        qn = ld_stream(&vl[(size_t)min((g + 8) >> 2, lastg) * N]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           // half hb of the position of lane pa + k -> slot pa + k
@@ -1327,7 +1372,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     }
   }
   if (!vok) {
-    R d_over = R(0);  // W of the hits past the list capacity, in scan order
+    R d_over = R(0);  // terms of the hits past the list capacity, in scan order
     int4 buf = make_int4(0, 0, 0, 0);  // last four hits, oldest in .x
     // Phase 1: the candidate test only.  Four candidates' positions are loaded
     // before any is tested, so their latencies overlap.
@@ -1338,7 +1383,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       if (r2 < rcutsq && k != i) {
         buf = make_int4(buf.y, buf.z, buf.w, k);
         cnt++;
-        if (cnt > kmax) d_over = d_over + sph_W<R>(pair_sqrt(r2), rinv, cw);
+        if (cnt > kmax) d_over = d_over + w_term(r2);
         else if ((cnt & 3) == 0) out[(size_t)((cnt >> 2) - 1) * N] = buf;
       }
     };
@@ -1369,7 +1414,8 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     // Phase 2: W over the stored hits with converged lanes (in phase 1 a wave
     // would evaluate W under a mask for every candidate any lane hits).  The
     // minimum image reproduces the strip segment's shift bit for bit.
-    d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+    // the particle itself (src/kernel_densities.cu:95-99): W(0) = 66 cw
+    d = sizeof(R) == 4 ? R(0) : sph_W<R>(R(0), rinv, cw);
     if (write_dens) {
       const int stored = min(cnt, kmax);
 #pragma unroll 1
@@ -1384,7 +1430,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (u >= m) break;
-            d = d + sph_W<R>(pair_sqrt((R)me.r2(Cand<R, DIM>{pj[u]})), rinv, cw);
+            d = d + w_term((R)me.r2(Cand<R, DIM>{pj[u]}));
           }
         } else {
           double xj[4], yj[4], zj[4];
@@ -1402,11 +1448,12 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
             pair_sep<R, DIM, WALL>(bx, xi, yi, zi, xj[u], yj[u], zj[u], r);
             const R r2 =
                 DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-            d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+            d = d + w_term(r2);
           }
         }
       }
-      d = d + d_over;
+      if constexpr (sizeof(R) == 4) d = cw * ((R(66) + d) + d_over);
+      else d = d + d_over;
     }
   }
   L.count[i] = cnt;
@@ -1462,15 +1509,17 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const R rv2 = (R)(rv * rv);
   int4 *__restrict__ out = VL.idx + i;
   const int cap = VL.cap;
+  const int pb = T.pbeg[sim];
   int cnt = 0;
   int4 buf = make_int4(0, 0, 0, 0);
   // The cut-off test of the density pass (float positions in the FP32 form),
-  // four candidates' loads in flight at a time.
+  // four candidates' loads in flight at a time.  Entries are indices within
+  // the simulation (k - pbeg), the offsets the passes' buffer loads take.
   CandSelf<R, DIM> me;
   me.init(S, i, P);
   auto test = [&](int k, const Cand<R, DIM> &c) {
     if ((R)me.r2(c) < rv2 && k != i) {
-      buf = make_int4(buf.y, buf.z, buf.w, k);
+      buf = make_int4(buf.y, buf.z, buf.w, k - pb);
       cnt++;
       if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
     }
@@ -1617,39 +1666,39 @@ __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<
     }
   }
   if (on_f) {
-    R Pi[6], Pj[6], diinv, djinv;
+    // Pi/d_i^2 + Pj/d_j^2, formed in each case's own registers (a common
+    // Pi, Pj pair made the compiler copy both into place for every pair)
+    R Ps[6], diinv, djinv;
     if (me.t == tj) {  // fluid - fluid
       diinv = me.dinv;
       djinv = rj.w;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        Pi[q] = me.pd[q];
-        Pj[q] = rj.q[q];
-      }
+      for (int q = 0; q < 6; ++q) Ps[q] = me.pd[q] + rj.q[q];
     } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure, keeps its c
       diinv = me.dinv;
       djinv = me.dinv;
       const R nkT = K.nkT, d2 = djinv * djinv;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) Pi[q] = me.pd[q];
+      R Pj[6];
       Pj[0] = (me.p - nkT * rj.q[0]) * d2;
       Pj[1] = (-nkT * rj.q[1]) * d2;
       Pj[2] = (me.p - nkT * rj.q[2]) * d2;
       Pj[3] = DIM == 3 ? (-nkT * rj.q[3]) * d2 : R(0);
       Pj[4] = DIM == 3 ? (-nkT * rj.q[4]) * d2 : R(0);
       Pj[5] = DIM == 3 ? (me.p - nkT * rj.q[5]) * d2 : R(0);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Ps[q] = me.pd[q] + Pj[q];
     } else {  // wall/colloid i: both sides use fluid j's d, p and c
       djinv = rj.w;
       diinv = djinv;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) Pi[q] = Pj[q] = rj.q[q];
+      for (int q = 0; q < 6; ++q) Ps[q] = rj.q[q] + rj.q[q];
     }
     const R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
-    const R axx = -gw * (Pi[0] + Pj[0]), axy = -gw * (Pi[1] + Pj[1]);
-    const R ayy = -gw * (Pi[2] + Pj[2]);
+    const R axx = -gw * Ps[0], axy = -gw * Ps[1];
+    const R ayy = -gw * Ps[2];
     if (DIM == 3) {
-      const R axz = -gw * (Pi[3] + Pj[3]), ayz = -gw * (Pi[4] + Pj[4]);
-      const R azz = -gw * (Pi[5] + Pj[5]);
+      const R axz = -gw * Ps[3], ayz = -gw * Ps[4];
+      const R azz = -gw * Ps[5];
       acc.f[0] += axx * e[0] + axy * e[1] + axz * e[2];
       acc.f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
       acc.f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
@@ -1857,17 +1906,19 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const int pb = __builtin_amdgcn_readfirstlane(T.pbeg[sim]);
     const unsigned oob =
         (unsigned)(__builtin_amdgcn_readfirstlane(iend) - pb) * (unsigned)sizeof(Rec);
+    // (a lane past its list re-reads its last index group and mask word: every
+    // lane walks to the wave's longest list, and new lines would cost traffic)
     const int lastg = max(vc - 1, 0) >> 2, lastw = max(vc - 1, 0) >> 5;
     // q: indices of the loader's current group of four; qn: the following
     // group.  m: hit bits of the current 32 entries, bit 0 the loader's entry
     // (masked to zero past the list); mn, mn2: the
     // following two words (unmasked until used; two ahead so that the word
     // taken over is never one still being loaded).
-    int4 q = vl[0];
-    int4 qn = vl[(size_t)min(1, lastg) * N];
-    unsigned m = vm[0];
-    unsigned mn = vm[(size_t)min(1, lastw) * N];
-    unsigned mn2 = vm[(size_t)min(2, lastw) * N];
+    int4 q = ld_stream(&vl[0]);
+    int4 qn = ld_stream(&vl[(size_t)min(1, lastg) * N]);
+    unsigned m = ld_stream(&vm[0]);
+    unsigned mn = ld_stream(&vm[(size_t)min(1, lastw) * N]);
+    unsigned mn2 = ld_stream(&vm[(size_t)min(2, lastw) * N]);
     if (vc < 32) m &= (1u << vc) - 1u;  // (vc = 0: no hits)
     // instruction k of a group load: the record of lane qa + src(k), chunk chk(k)
     auto src = [](int k) { return k / kPer; };
@@ -1904,12 +1955,12 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
       m >>= 1;
       if ((k & 3) == 0) {
         q = qn;
-        qn = vl[(size_t)min((k + 4) >> 2, lastg) * N];
+        qn = ld_stream(&vl[(size_t)min((k + 4) >> 2, lastg) * N]);
       }
       if ((k & 31) == 0) {
         m = mn;
         mn = mn2;
-        mn2 = vm[(size_t)min((k + 64) >> 5, lastw) * N];
+        mn2 = ld_stream(&vm[(size_t)min((k + 64) >> 5, lastw) * N]);
         const int left = vc - k;
         if (left < 32) m &= left > 0 ? (1u << left) - 1u : 0u;
       }
@@ -1917,7 +1968,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     auto load_at = [&](int k, u4(&v)[kLoads]) {
       const int c = k & 3;  // (known at compile time, see kTrip)
       const int j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
-      quad_load((m & 1u) ? (unsigned)(j - pb) * (unsigned)sizeof(Rec) : oob, v);
+      quad_load((m & 1u) ? (unsigned)j * (unsigned)sizeof(Rec) : oob, v);
     };
     // pend: hit bits of the entries loaded but not yet evaluated (bit 0: the
     // next to evaluate)
@@ -3689,6 +3740,8 @@ class DeviceImpl final : public Device {
   bool coll_lds_ = true;  // every simulation's colloids fit the pair pass's LDS stage
   int nc_ = 3, cur_ = 0;
   int scan_sum_tiles_ = kScanSumTiles;
+  // Threads per workgroup of the density pass: kBlock, or 1024 (four tiles
+  // per workgroup; only with kBlock tiles).  SPH_DENSITY_BLOCK selects it.
   int block_ = kBlock, xcd_ = 1, nblk_ = 0, max_parts_ = 1;
   // Verlet lists (used by run_steps): enabled when skin > 0 and every periodic
   // axis of every simulation has at least 5 cells.

@@ -162,6 +162,11 @@ def _run_batch(b, g, steps, samples, autotune, targets, info, log):
             continue
         wall = 0.5 * (o[:, j, 3] - o[:, j, 4])
         targets[i, 0] = float(np.mean(wall))
+        # how settled the average is: the two halves of the window, relative
+        h = len(wall) // 2
+        if h:
+            info["settle"][i] = float(abs(np.mean(wall[h:]) - np.mean(wall[:h]))
+                                      / max(abs(targets[i, 0]), 1e-300))
         targets[i, 1] = float(np.mean(o[:, j, 1]))
         targets[i, 2] = _circ_half(o[:, j, 7])
     if log:
@@ -179,7 +184,8 @@ def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000
     is the device time of the batches (upload, steps, observables), without
     the host set-up of the inputs."""
     targets = np.full((len(points), 3), np.nan)
-    info = dict(batches=0, particle_steps=0, seconds=0.0, particles=[], failed=0)
+    info = dict(batches=0, particle_steps=0, seconds=0.0, particles=[], failed=0,
+                settle=np.full(len(points), np.nan))
     order = sorted(range(len(points)), key=lambda i: -points[i]["n"])
     with tempfile.TemporaryDirectory() as tmpdir:
         b, g, total = None, [], 0
