@@ -119,6 +119,11 @@ struct SimTable {
   const int *coll_off;    // [nsim + 1] first colloid
   const int *coll_sim;    // [total colloids] owning simulation
   const int *wrange;      // [2 nsim + 1] wall-list ranges: wall w of sim s is [2s+w, 2s+w+1)
+  // [nsim] nonzero: the simulation has failed (a particle left the grid) and
+  // is held: from then on no kernel moves its particles or colloids, reorders
+  // or rebuilds it, or evaluates its pairs, so a failed member of a batch costs
+  // nothing and its state stays as it was when it failed.  Set by the hash.
+  int *halt;
   int nsim, nblk, ntotal, xcd;
 };
 
@@ -876,7 +881,10 @@ __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__res
   if (i < T.pbeg[sim + 1]) {
     bool ok;
     c = T.cell_off[sim] + cell_of(P, x[i], y[i], z[i], ok);
-    if (!ok) atomicOr(&flag[sim], 1);
+    if (!ok) {
+      atomicOr(&flag[sim], 1);
+      atomicOr(&T.halt[sim], 1);
+    }
     cell[i] = c;
   }
   const int s = count_into_cell(c, count);
@@ -893,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__res
   if (any && !*any) return;
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int k = T.blk_first[b] + threadIdx.x;
-  if (k >= T.pbeg[sim + 1] || (rebuild && !rebuild[sim])) return;
+  if (k >= T.pbeg[sim + 1] || (rebuild && !rebuild[sim]) || T.halt[sim]) return;
   int c = cell[k];
   if (c < 0) return;
   int p = cs[c] + slot[k];
@@ -903,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__res
 
 // Final position = cell start + number of particles of the same cell with a
 // smaller ID, i.e. the stable (cell, ID) order.  Gathers the moving state.
-// A simulation that is not rebuilt keeps its order: with copy_others its state
+// A simulation that is not rebuilt (or is held) keeps its order: with copy_others its state
 // is copied position for position into the other buffer set (a full reorder
 // flips the sets); without, it is left alone (a Verlet step copies the rebuilt
 // simulations back instead, k_copy_back).
@@ -916,17 +924,20 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int p = T.blk_first[b] + threadIdx.x;
   if (p >= T.pbeg[sim + 1]) return;
-  if (rebuild && !rebuild[sim]) {
+  if ((rebuild && !rebuild[sim]) || T.halt[sim]) {
     if (!copy_others) return;
-    dst.x[p] = src.x[p];
-    dst.y[p] = src.y[p];
-    dst.z[p] = src.z[p];
+    const double xp = src.x[p], yp = src.y[p], zp = src.z[p];
+    dst.x[p] = xp;
+    dst.y[p] = yp;
+    dst.z[p] = zp;
+    store_pos(dst, T.P[sim], p, xp, yp, zp);
 #pragma unroll
     for (int a = 0; a < 3; ++a) dst.v[a][p] = src.v[a][p];
     for (int a = 0; a < nc; ++a) dst.c[a][p] = src.c[a][p];
     dst.mass[p] = src.mass[p];
     dst.type[p] = src.type[p];
     dst.id[p] = src.id[p];
+    if (newpos) newpos[p] = p;
     return;
   }
   const int k = stage_src[p];
@@ -964,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_back(SimTable T, Particles<R> s
   if (blockIdx.x == 0 && threadIdx.x == 0) *any = 0;
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const int p = T.blk_first[b] + threadIdx.x;
-  if (p >= T.pbeg[sim + 1] || !rebuild[sim]) return;
+  if (p >= T.pbeg[sim + 1] || !rebuild[sim] || T.halt[sim]) return;
   dst.x[p] = src.x[p];
   dst.y[p] = src.y[p];
   dst.z[p] = src.z[p];
@@ -1173,7 +1184,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     // After a Verlet step's list build (the previous launch): a simulation
     // whose list overflowed is rebuilt again next step (no thread of this pass
     // reads the flags).
-    const int of = V.overflow[sim];
+    const int of = V.overflow[sim] && !T.halt[sim];
     V.rebuild[sim] = of ? 1 : 0;
     if (of) atomicOr(V.any, 1);  // (the next step's flag)
     V.overflow[sim] = 0;
@@ -1184,6 +1195,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
   const int iend = T.pbeg[sim + 1];
   const int i0 = T.blk_first[b] + threadIdx.x;
   if (i0 == 0) *L.qlen = 0;  // the overflow queue of the following pair pass
+  if (T.halt[sim]) return;
   const bool live = i0 < iend;
   // Verlet walk: every lane of a wave takes part in the pair loads, so the
   // lanes past the simulation's end stay (aliasing its last particle for their
@@ -1495,7 +1507,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     if (s < T.nsim) set_step_of(T, steps, mag, step, s);
   }
   const int b = tile_of(T), sim = T.blk_sim[b];
-  if (!V.rebuild[sim]) return;
+  if (!V.rebuild[sim] || T.halt[sim]) return;
   const int i = T.blk_first[b] + threadIdx.x;
   if (i == T.pbeg[sim]) atomicAdd(&V.rebuilds[sim], 1ull);
   if (i >= T.pbeg[sim + 1]) return;
@@ -1830,6 +1842,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     const PairRec<R, DIM> *__restrict__ rec,
     NList L, int store_gv, int write_dcdt, int with_ext, int kick, VList VL) {
   const int b = tile_of(T), sim = T.blk_sim[b];
+  if (T.halt[sim]) return;  // (the whole workgroup: one simulation)
   const int iend = T.pbeg[sim + 1];
   const int i0 = T.blk_first[b] + threadIdx.x;
   const bool live = i0 < iend;
@@ -2306,7 +2319,7 @@ __device__ __forceinline__ void quat_mul(const double c[4], const double d[4], d
 // with |omega| == 0 keeps its orientation (the reference divides by zero).
 __device__ __forceinline__ void coll_vv1_of(const SimTable &T, Bodies B, int i) {
   const sph_params_t &P = T.P[T.coll_sim[i]];
-  if (P.coll_move != 0) return;
+  if (P.coll_move != 0 || T.halt[T.coll_sim[i]]) return;
   double **C = B.s;
   const bool d3 = P.dim == 3;
   const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
@@ -2376,7 +2389,7 @@ __global__ __launch_bounds__(kBlock) void k_vv1(SimTable T, Particles<R> S, Deri
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
-  if (i >= T.pbeg[sim + 1]) return;
+  if (i >= T.pbeg[sim + 1] || T.halt[sim]) return;
   const int t = S.type[i];
   if (t > 2) return;
   double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
@@ -2427,7 +2440,7 @@ __global__ __launch_bounds__(kBlock) void k_vv2(SimTable T, Particles<R> S, Deri
   const int b = blockIdx.x, sim = T.blk_sim[b];
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
-  if (i >= T.pbeg[sim + 1] || S.type[i] != 0) return;
+  if (i >= T.pbeg[sim + 1] || T.halt[sim] || S.type[i] != 0) return;
   const R hdt = (R)(0.5 * P.dt);
   const R hdtm = hdt / S.mass[i];
   S.v[0][i] = S.v[0][i] + hdtm * D.f[0][i];
@@ -2449,7 +2462,7 @@ __global__ __launch_bounds__(kBlock) void k_coll_place(SimTable T, Particles<R> 
   const int ci = B.cowner[j];  // colloid of the batch
   const int sim = T.coll_sim[ci];
   const sph_params_t &P = T.P[sim];
-  if (P.coll_move != 0) return;
+  if (P.coll_move != 0 || T.halt[sim]) return;
   double *const *C = B.s;
   double px, py, pz = 0.0;
   if (P.dim == 2) {
@@ -2490,7 +2503,7 @@ __global__ void k_coll_vv2(SimTable T, Bodies B, int ncoll) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncoll) return;
   const sph_params_t &P = T.P[T.coll_sim[i]];
-  if (P.coll_move != 0) return;
+  if (P.coll_move != 0 || T.halt[T.coll_sim[i]]) return;
   double **C = B.s;
   const double hdtm = 0.5 * P.dt / P.coll_mass, hdtI = 0.5 * P.dt / P.coll_I;
   C[C_VX][i] += hdtm * C[C_FX][i];
@@ -2749,6 +2762,7 @@ class DeviceImpl final : public Device {
     HIPCHK(count_.alloc(ncells_)); HIPCHK(cs_.alloc(ncells_ + 1));
     HIPCHK(tsum_.alloc((ncells_ + kScanTile - 1) / kScanTile));
     HIPCHK(flag_.alloc(S));
+    HIPCHK(halt_.alloc(S));
     // Neighbour-list capacity: 1.5x the mean number of particles inside the
     // cut-off sphere (disc in 2D) at the initial number density, plus 64, for
     // the densest simulation of the batch.
@@ -2885,6 +2899,7 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemset(count_.p, 0, ncells_ * sizeof(int)));
     HIPCHK(hipMemset(cs_.p, 0, (ncells_ + 1) * sizeof(int)));
     HIPCHK(hipMemset(flag_.p, 0, S * sizeof(int)));
+    HIPCHK(hipMemset(halt_.p, 0, S * sizeof(int)));
     HIPCHK(hipMemset(fwall_.p, 0, 6 * S * sizeof(double)));
     HIPCHK(hipMemset(steps_.p, 0, S * sizeof(int)));
     HIPCHK(hipMemset(mag_.p, 0, 3 * S * sizeof(double)));
@@ -3450,6 +3465,9 @@ class DeviceImpl final : public Device {
   int upload_field(int sim, const HostState &hs, const std::string &name,
                    std::string &err) override {
     HIPCHK(hipStreamSynchronize(stream_));
+    // new host state releases a held simulation (the next hash holds it again
+    // if a particle is still outside the grid)
+    HIPCHK(hipMemset(halt_.p + sim, 0, sizeof(int)));
     rec_valid_ = false;
     int rc = refresh_ids(err);
     if (rc) return rc;
@@ -3629,6 +3647,7 @@ class DeviceImpl final : public Device {
     T.coll_off = coll_off_.p;
     T.coll_sim = coll_sim_.p;
     T.wrange = wrange_.p;
+    T.halt = halt_.p;
     T.nsim = (int)sims_.size();
     T.nblk = nblk_;
     T.ntotal = N_;
@@ -3775,6 +3794,7 @@ class DeviceImpl final : public Device {
   PairRec<R, DIM> *recs() { return reinterpret_cast<PairRec<R, DIM> *>(rec_.p); }
   bool rec_valid_ = false;
   DevBuf<int> cell_, slot_, rank_, stage_src_, stage_id_, sorted_cell_, count_, cs_, tsum_, flag_;
+  DevBuf<int> halt_;  // [nsim] SimTable::halt
   DevBuf<int4> nl_idx_;
   DevBuf<int> nl_count_, nl_queue_, nl_qlen_;
   int kmax_ = 0;

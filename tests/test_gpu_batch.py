@@ -146,7 +146,8 @@ def test_batch_of_one_equals_system(tmp_path):
 
 def test_failed_simulation_is_flagged_and_isolated(tmp_path):
     """A particle moved beyond a wall band in one simulation flags that
-    simulation only; the others reproduce their stand-alone runs bit for bit."""
+    simulation only; the others reproduce their stand-alone runs bit for bit.
+    The failed simulation is held: further steps leave its state as it was."""
     singles = _systems("2d", str(tmp_path), sph_system.FP64, "s")
     members = _systems("2d", str(tmp_path), sph_system.FP64, "b")
     y = members[1].get("y")
@@ -165,3 +166,13 @@ def test_failed_simulation_is_flagged_and_isolated(tmp_path):
         s.step(3)
         for n in _names(s.params):
             np.testing.assert_array_equal(b.get(k, n), s.get(n), err_msg=f"sim {k} {n}")
+    held = {n: b.get(1, n) for n in ("x", "y", "vx", "vy", "cxy")}
+    b.step(4)
+    assert b.failed()[1] & 1
+    for n, v in held.items():
+        np.testing.assert_array_equal(b.get(1, n), v, err_msg=f"held sim {n}")
+    for k in (0, 2, 3):
+        s = singles[k]
+        s.step(4)
+        for n in ("x", "vx", "fx"):
+            np.testing.assert_array_equal(b.get(k, n), s.get(n), err_msg=f"sim {k} {n} later")
