@@ -173,6 +173,14 @@ struct VList {
   // of word e / 32 (column-major, words * N), so the pair pass visits exactly
   // the neighbours without a second index list.
   unsigned *mask;
+  // FP32 form, staged density walk (null when off): each entry's window slot,
+  // eight 16-bit slots per group ((cap / 8) * N groups, column-major); per
+  // window tile its window (kWinInts ints; see window_build); per simulation
+  // the index of its first window
+  uint4 *slot;
+  int *win;
+  const int *wbase;
+  int stage_cap;    // positions per window (<= kStageCap)
   int cap;
   float skin_frac;  // skin / rcut
 };
@@ -257,6 +265,11 @@ __device__ __forceinline__ unsigned fix_enc(double x, double s, double o) {
 // that lists the same neighbour).
 __device__ __forceinline__ unsigned ld_stream(const unsigned *p) {
   return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ uint4 ld_stream(const uint4 *p) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ int4 ld_stream(const int4 *p) {
   typedef int v4i __attribute__((ext_vector_type(4)));
@@ -539,6 +552,119 @@ __device__ __forceinline__ void for_each_reach(const sph_params_t &P, const int 
     const int base = (z * Ny + y) * Nx;
     fn(cs[base + (part ? b0 : a0)], cs[base + (part ? b1 : a1)], part ? sb : sa, sy, sz);
   }
+}
+
+// Windows of the staged FP32 density walk.  At a list build each window tile
+// (kWinTile consecutive particles of a simulation, from its first; a workgroup
+// of the passes lies inside one) gets a window: the sorted-index ranges of the
+// cells within one cell per axis of the cells its particles occupy (periodic
+// images folded in, rows beyond a wall dropped), merged.  Every list entry also
+// gets its slot in the window, or kNoSlot for the few entries that a build's
+// reach adds from the second ring of cells.  The density pass copies the
+// window's fixed-point positions into LDS once per workgroup and reads each
+// entry's position from there: the gathers no longer go through the vector L1
+// one cache line per lane, and a position is fetched once per tile instead of
+// once per particle that lists it.  Windows do not depend on the launch shape.
+constexpr int kWinTile = 256;                 // particles per window tile
+constexpr int kWinRanges = 24;                // merged index ranges per window
+constexpr int kWinInts = 2 + 2 * kWinRanges;  // [ranges | -1, positions, (first, count)...]
+constexpr int kStageCap = 3584;               // positions per window (3 x 4 B each in LDS)
+constexpr unsigned kNoSlot = 0xFFFFu;
+
+// The window of a tile whose particles occupy cells c0 .. c1 (the simulation's
+// cell indices, sorted order), built by the 64 lanes of one wavefront: w[0] =
+// the number of merged ranges, or -1 when the tile spans rows of cells whose
+// neighbour rows give more than 64 unmerged ranges, or the window has more than
+// kWinRanges ranges or `cap` positions; w[1] = positions; then the (first global
+// index, count) of each range in increasing order; woff[g] = the slot of the
+// first index of range g.
+template <int DIM, bool WALL>
+__device__ void window_build(const sph_params_t &P, const int *__restrict__ cs, int c0, int c1,
+                             int cap, int *w, int *woff) {
+  constexpr int kRows = DIM == 3 ? 9 : 3;
+  const int lane = threadIdx.x & 63;
+  const int Nx = P.Ncells[0], Ny = P.Ncells[1], Nz = DIM == 3 ? P.Ncells[2] : 1;
+  const int r0 = c0 / Nx, r1 = c1 / Nx, nseg = r1 - r0 + 1;
+  const bool fits = c0 >= 0 && c1 >= c0 && nseg * kRows <= 32;
+  // lane L < 32 makes the (up to two) ranges of segment row r0 + L / kRows and
+  // its neighbour row L % kRows
+  int2 a = make_int2(INT_MAX, INT_MAX), b = a;
+  if (fits && lane < nseg * kRows) {
+    const int r = r0 + lane / kRows, nr = lane % kRows;
+    const int xa = r == r0 ? c0 - r * Nx : 0, xb = r == r1 ? c1 - r * Nx : Nx - 1;
+    int y = r % Ny + nr % 3 - 1, z = DIM == 3 ? r / Ny + nr / 3 - 1 : 0;
+    bool row = true;
+    if (WALL) row = y >= 0 && y < Ny; else y = (y + Ny) % Ny;
+    if (DIM == 3) z = (z + Nz) % Nz;
+    if (row) {
+      const int base = (z * Ny + y) * Nx, x0 = xa - 1, x1 = xb + 1;
+      int p0 = x0, p1 = x1, q0 = -1, q1 = -1;  // cells p0..p1, then q0..q1, of the row
+      if (x1 - x0 + 1 >= Nx) { p0 = 0; p1 = Nx - 1; }
+      else if (x0 < 0) { p0 = x0 + Nx; p1 = Nx - 1; q0 = 0; q1 = x1; }
+      else if (x1 >= Nx) { p1 = Nx - 1; q0 = 0; q1 = x1 - Nx; }
+      const int s0 = cs[base + p0], e0 = cs[base + p1 + 1];
+      if (e0 > s0) a = make_int2(s0, e0);
+      if (q0 >= 0) {
+        const int s1 = cs[base + q0], e1 = cs[base + q1 + 1];
+        if (e1 > s1) b = make_int2(s1, e1);
+      }
+    }
+  }
+  // candidate L = range L & 1 of lane L / 2
+  const int src = lane >> 1;
+  const int ax = __shfl(a.x, src), ay = __shfl(a.y, src), bx = __shfl(b.x, src), by = __shfl(b.y, src);
+  const int cs0 = (lane & 1) ? bx : ax, ce0 = (lane & 1) ? by : ay;
+  const bool live = cs0 != INT_MAX;
+  // rank by (first index, candidate), then each lane takes the candidate of its rank
+  int rank = 0;
+  for (int o = 0; o < 64; ++o) {
+    const int so = __shfl(cs0, o);
+    rank += (so < cs0 || (so == cs0 && o < lane)) ? 1 : 0;
+  }
+  const int n = __popcll(__ballot(live));
+  int s1 = INT_MAX, e1 = INT_MAX;
+  for (int o = 0; o < 64; ++o) {
+    const int ro = __shfl(rank, o), so = __shfl(cs0, o), eo = __shfl(ce0, o);
+    if (__shfl(live ? 1 : 0, o) && ro == lane) { s1 = so; e1 = eo; }
+  }
+  // merge: a range starts a group unless it begins at or before the furthest
+  // end of the ranges before it
+  int pm = lane < n ? e1 : INT_MIN;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(pm, o);
+    if (lane >= o) pm = max(pm, t);
+  }
+  int prev = __shfl_up(pm, 1);
+  if (lane == 0) prev = INT_MIN;
+  const bool head = lane < n && s1 > prev;
+  const unsigned long long heads = __ballot(head);
+  const int m = __popcll(heads);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const int g = __popcll(heads & upto) - 1;
+  const bool last = lane < n && (lane == n - 1 || ((heads >> (lane + 1)) & 1ull));
+  const int hl = 63 - __clzll(heads & upto);  // head lane of this lane's group
+  const int gs = __shfl(s1, hl < 0 ? 0 : hl);
+  const int len = last ? pm - gs : 0;
+  int incl = len;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  const int total = __shfl(incl, 63);
+  if (last && g < kWinRanges) {
+    w[2 + 2 * g] = gs;
+    w[3 + 2 * g] = len;
+    woff[g] = incl - len;
+  }
+  if (lane == 0) {
+    w[0] = fits && m <= kWinRanges && total <= cap ? m : -1;
+    w[1] = total;
+  }
+}
+
+// Window tile of workgroup tile b and the index of its window.
+__device__ __forceinline__ int window_of(const SimTable &T, const int *wbase, int b, int sim) {
+  return wbase[sim] + (T.blk_first[b] - T.pbeg[sim]) / kWinTile;
 }
 
 // r_ij of a candidate inside a strip segment: (x_i - x_j) + s in double, rounded to R.
@@ -1272,21 +1398,113 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
           mw = 0u;
         }
       };
-      // The positions of the next group are in flight while a group is
-      // evaluated, and its indices one group further ahead.  (Each lane leaves
-      // the loop after its own list: finished lanes issue no loads; the group
-      // counter is the same in every lane still in it.)
-      int4 qn = index_group(1);
-      uint4 cur[4];
-      group_load(index_group(0), 0, cur);
+      // (uniform: the tile's window, or -1 for the unstaged walk)
+      const int *__restrict__ gw =
+          VL.slot ? VL.win + (size_t)window_of(T, VL.wbase, b, sim) * kWinInts : nullptr;
+      const int nwin = gw ? gw[0] : -1;
+      if (nwin >= 0) {
+        // Staged walk: the window's positions are copied into LDS (coalesced
+        // runs of the sorted order), then each entry's position is read from
+        // its slot; entries of the second ring of cells (kNoSlot) take their
+        // index from the list and their position from memory.  The same
+        // positions in the same order as the unstaged walk: the same sums.
+        __shared__ unsigned wx[kStageCap], wy[kStageCap], wz[DIM == 3 ? kStageCap : 1];
+        __shared__ int wr[kWinInts];
+        if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
+        __syncthreads();
+        // slot p of the window: range r (held from the thread's previous slot,
+        // which is lower) at offset p - ro; four loads in flight per thread
+        const int total = wr[1];
+        int r = 0, ro = 0;
 #pragma unroll 1
-      for (int g = 0; g < vc; g += 4) {
-        uint4 nxt[4];
-        group_load(qn, g + 4, nxt);
-        qn = index_group((g >> 2) + 2);
-        group_eval(g, cur);
+        for (int p0 = (int)threadIdx.x; p0 < total; p0 += 4 * (int)blockDim.x) {
+          uint4 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+          for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u * (int)blockDim.x;
+            if (p < total) {
+              while (p >= ro + wr[3 + 2 * r]) ro += wr[3 + 2 * r++];
+              q[u] = S.pq[wr[2 + 2 * r] + (p - ro)];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u * (int)blockDim.x;
+            if (p < total) {
+              wx[p] = q[u].x;
+              wy[p] = q[u].y;
+              if (DIM == 3) wz[p] = q[u].z;
+            }
+          }
+        }
+        __syncthreads();
+        const uint4 *__restrict__ sl = VL.slot + i;
+        const int capq = VL.cap >> 3;
+        auto slot_group = [&](int gq) { return ld_stream(&sl[(size_t)min(gq, capq - 1) * N]); };
+        uint4 sg = slot_group(0);
+#pragma unroll 1
+        for (int g = 0; g < vc; g += 8) {
+          const uint4 sn = slot_group((g >> 3) + 1);
+          const unsigned w4[4] = {sg.x, sg.y, sg.z, sg.w};
+          // every slot is read (clamped into the window: entries past the list
+          // hold any slot and are masked below); entries of the second ring of
+          // cells are fetched from memory afterwards
+          uint4 c[8];
+          bool far = false;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+            const unsigned sc = min(sk, (unsigned)(kStageCap - 1));
+            c[u] = make_uint4(wx[sc], wy[sc], DIM == 3 ? wz[sc] : 0u, 1u);
+            far |= sk == kNoSlot && g + u < vc;
+          }
+          if (far) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+              if (sk == kNoSlot && g + u < vc) {
+                const int4 q4 = vl[(size_t)((g + u) >> 2) * N];
+                const int cq = (g + u) & 3;
+                const int j = cq == 0 ? q4.x : cq == 1 ? q4.y : cq == 2 ? q4.z : q4.w;
+                c[u] = __builtin_bit_cast(
+                    uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((unsigned)j * 16u), 0, 0));
+              }
+            }
+          }
+          // (branch-free: the term of every entry is formed and a miss adds
+          // +0, which leaves the sum bitwise as if the miss were skipped)
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const R r2 = (R)me.r2(Cand<R, DIM>{c[u]});
+            const R t = w_term(r2);
+            const bool hit = g + u < vc && r2 < rcutsq;
+            tsum = tsum + (hit ? t : R(0));
+            mw |= (hit ? 1u : 0u) << ((g + u) & 31);
+          }
+          if (((g + 8) & 31) == 0 || g + 8 >= vc) {
+            VL.mask[(size_t)(g >> 5) * N + i] = mw;
+            cnt += __builtin_popcount(mw);
+            mw = 0u;
+          }
+          sg = sn;
+        }
+      } else {
+        // The positions of the next group are in flight while a group is
+        // evaluated, and its indices one group further ahead.  (Each lane
+        // leaves the loop after its own list: finished lanes issue no loads;
+        // the group counter is the same in every lane still in it.)
+        int4 qn = index_group(1);
+        uint4 cur[4];
+        group_load(index_group(0), 0, cur);
+#pragma unroll 1
+        for (int g = 0; g < vc; g += 4) {
+          uint4 nxt[4];
+          group_load(qn, g + 4, nxt);
+          qn = index_group((g >> 2) + 2);
+          group_eval(g, cur);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+        }
       }
       if (!live) return;
       if (vok) d = cw * (R(66) + tsum);  // (W(0) = 66 cw: the particle itself)
@@ -1510,10 +1728,50 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   if (!V.rebuild[sim] || T.halt[sim]) return;
   const int i = T.blk_first[b] + threadIdx.x;
   if (i == T.pbeg[sim]) atomicAdd(&V.rebuilds[sim], 1ull);
-  if (i >= T.pbeg[sim + 1]) return;
   const sph_params_t &P = T.P[sim];
   const int *__restrict__ cs = csg + T.cell_off[sim];
   const int N = T.ntotal;
+  const int iend = T.pbeg[sim + 1];
+  // FP32 form with the staged density walk: the window of this workgroup's
+  // window tile, from the cells of the tile's first and last particle (the
+  // reorder of this step sorted them by these cells), before any lane leaves;
+  // the tile's first workgroup stores it
+  constexpr bool kSlots = sizeof(R) == 4;
+  __shared__ int wwin[kSlots ? kWinInts : 1];
+  __shared__ int woff[kSlots ? kWinRanges : 1];
+  const bool slots = kSlots && VL.slot;
+  if (slots) {
+    if (threadIdx.x < 64) {
+      const int f = T.pbeg[sim] + (T.blk_first[b] - T.pbeg[sim]) / kWinTile * kWinTile;
+      const int l = min(f + kWinTile, iend) - 1;
+      bool ok;
+      const int c0 = cell_of(P, S.x[f], S.y[f], DIM == 3 ? S.z[f] : 0.0, ok);
+      const int c1 = cell_of(P, S.x[l], S.y[l], DIM == 3 ? S.z[l] : 0.0, ok);
+      window_build<DIM, WALL>(P, cs, c0, c1, VL.stage_cap, wwin, woff);
+    }
+    __syncthreads();
+    if (T.blk_first[b] % kWinTile == T.pbeg[sim] % kWinTile && threadIdx.x < kWinInts)
+      VL.win[(size_t)window_of(T, VL.wbase, b, sim) * kWinInts + threadIdx.x] = wwin[threadIdx.x];
+  }
+  if (i >= iend) return;
+  const bool staged = slots && wwin[0] >= 0;
+  const int wm = staged ? wwin[0] : 0;
+  // the window range of the last slot looked up: [wlo, whi), slot = k + wsb
+  int wlo = 0, whi = 0, wsb = 0;
+  auto slot_of = [&](int k) -> unsigned {
+    if (k >= wlo && k < whi) return (unsigned)(k + wsb);
+    int lo = 0, hi = wm - 1;
+    while (lo < hi) {  // the last range whose first index is <= k
+      const int mid = (lo + hi + 1) >> 1;
+      if (wwin[2 + 2 * mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const int f = wwin[2 + 2 * lo], n = wwin[3 + 2 * lo];
+    if (k < f || k >= f + n) return kNoSlot;
+    wlo = f;
+    whi = f + n;
+    wsb = woff[lo] - f;
+    return (unsigned)(k + wsb);
+  };
   const double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
@@ -1524,6 +1782,10 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const int pb = T.pbeg[sim];
   int cnt = 0;
   int4 buf = make_int4(0, 0, 0, 0);
+  // slots of the current group of eight entries, two per word (entry 2k + h
+  // in bits 16 h of word k)
+  unsigned sw[4] = {0u, 0u, 0u, 0u};
+  uint4 *__restrict__ sout = slots ? VL.slot + i : nullptr;
   // The cut-off test of the density pass (float positions in the FP32 form),
   // four candidates' loads in flight at a time.  Entries are indices within
   // the simulation (k - pbeg), the offsets the passes' buffer loads take.
@@ -1532,8 +1794,16 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   auto test = [&](int k, const Cand<R, DIM> &c) {
     if ((R)me.r2(c) < rv2 && k != i) {
       buf = make_int4(buf.y, buf.z, buf.w, k - pb);
+      if (staged) {
+        const int e = cnt & 7;
+        sw[e >> 1] |= slot_of(k) << (16 * (e & 1));
+      }
       cnt++;
       if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
+      if (staged && (cnt & 7) == 0) {
+        if (cnt <= cap) sout[(size_t)((cnt >> 3) - 1) * N] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
+        sw[0] = sw[1] = sw[2] = sw[3] = 0u;
+      }
     }
   };
   for_each_reach<DIM, WALL>(P, cs, xi, yi, zi, cx, cy, cz, rv, [&](int k0, int k1, double sx,
@@ -1560,6 +1830,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
     out[(size_t)(cnt >> 2) * N] = buf;
   }
+  if (staged && (cnt & 7) && cnt < cap)
+    sout[(size_t)(cnt >> 3) * N] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
   VL.count[i] = cnt;
   // the displacement checks compare float positions with this reference
   V.ref[i] = make_float4((float)xi, (float)yi, (float)zi, 0.0f);
@@ -2799,11 +3071,33 @@ class DeviceImpl final : public Device {
       vmean = std::max(vmean, p.N / vol * (p.dim == 3 ? 4.0 / 3.0 * M_PI * rv * rv * rv
                                                        : M_PI * rv * rv));
     }
-    vcap_ = ((int)(1.5 * vmean) + 64 + 3) / 4 * 4;
+    // (a multiple of 8: the staged walk's slots come in groups of eight)
+    vcap_ = ((int)(1.5 * vmean) + 64 + 7) / 8 * 8;
     if (const char *e = std::getenv("SPH_VERLET_CAPACITY"))
-      vcap_ = std::max(4, (std::atoi(e) + 3) / 4 * 4);
+      vcap_ = std::max(8, (std::atoi(e) + 7) / 8 * 8);
+    // Staged FP32 density walk (SPH_DENSITY_STAGE=0 turns it off; SPH_STAGE_CAP
+    // lowers the positions a window may hold, so that more tiles take the
+    // unstaged walk; results are the same either way)
+    stage_ = sizeof(R) == 4 && verlet_ok_;
+    if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_ = stage_ && std::atoi(e) != 0;
+    stage_cap_ = kStageCap;
+    if (const char *e = std::getenv("SPH_STAGE_CAP"))
+      stage_cap_ = std::min(kStageCap, std::max(0, std::atoi(e)));
     if (verlet_ok_) {
       HIPCHK(vl_idx_.alloc((size_t)(vcap_ / 4) * N_));
+      if (stage_) {
+        HIPCHK(vl_slot_.alloc((size_t)(vcap_ / 8) * N_));
+        // window tiles of kWinTile particles from each simulation's first
+        std::vector<int> wb(S);
+        int nw = 0;
+        for (int k = 0; k < S; ++k) {
+          wb[k] = nw;
+          nw += (sims_[k].n + kWinTile - 1) / kWinTile;
+        }
+        HIPCHK(wbase_.alloc(S));
+        HIPCHK(hipMemcpy(wbase_.p, wb.data(), S * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(vl_win_.alloc((size_t)std::max(nw, 1) * kWinInts));
+      }
       HIPCHK(vl_mask_.alloc((size_t)((vcap_ + 31) / 32) * N_));
       HIPCHK(vl_count_.alloc(N_));
       HIPCHK(ref4_.alloc(N_));
@@ -3681,12 +3975,18 @@ class DeviceImpl final : public Device {
   NList nlist() { return NList{nl_idx_.p, nl_count_.p, nl_queue_.p, nl_qlen_.p, kmax_}; }
   // The Verlet lists while a Verlet step runs; no lists otherwise.
   VList vlist() {
-    VList v{nullptr, nullptr, nullptr, 0, skin_frac_};
+    VList v{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, skin_frac_};
     if (in_verlet_) {
       v.idx = vl_idx_.p;
       v.count = vl_count_.p;
       v.mask = vl_mask_.p;
       v.cap = vcap_;
+      if (stage_) {
+        v.slot = vl_slot_.p;
+        v.win = vl_win_.p;
+        v.wbase = wbase_.p;
+        v.stage_cap = stage_cap_;
+      }
     }
     return v;
   }
@@ -3768,6 +4068,11 @@ class DeviceImpl final : public Device {
   float skin_frac_ = 0.04f;
   int vcap_ = 0;
   DevBuf<int4> vl_idx_;
+  DevBuf<uint4> vl_slot_;  // VList::slot
+  DevBuf<int> vl_win_;     // VList::win
+  DevBuf<int> wbase_;      // VList::wbase
+  bool stage_ = false;
+  int stage_cap_ = kStageCap;
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
   DevBuf<int> any_;  // [1] batch rebuild flag (VerletCtl::any)

@@ -381,3 +381,34 @@ def test_observables(case, tmp_path):
     b = o.observables()
     both_nan = np.isnan(a) & np.isnan(b)
     np.testing.assert_allclose(a[~both_nan], b[~both_nan], rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["colloids3d", "sine3d", "couette2d"])
+def test_staged_density_walk_is_the_unstaged_one(case, tmp_path, monkeypatch):
+    """The FP32 density walk reads neighbour positions from a per-tile LDS
+    window (SPH_DENSITY_STAGE=0 reads them from memory; SPH_STAGE_CAP=400
+    leaves the tiles with larger windows to the unstaged walk).  The same
+    positions in the same order: bitwise the same trajectory, with list
+    rebuilds (a skin of 0.01 rcut) inside the run."""
+    monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
+    cfg = {"couette2d": lambda: helpers.cfg_couette(n=48, Vwall_bottom=-2.0, Vwall_top=2.0),
+           "colloids3d": lambda: helpers.cfg_colloids_3d(n=24),
+           "sine3d": lambda: helpers.cfg_sine(n=24)}[case]()
+    names = ("x", "y", "vx", "vy", "dens", "fx", "fy", "cxy")
+    outs = []
+    for mode in ({}, {"SPH_DENSITY_STAGE": "0"}, {"SPH_STAGE_CAP": "400"}):
+        for k in ("SPH_DENSITY_STAGE", "SPH_STAGE_CAP"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in mode.items():
+            monkeypatch.setenv(k, v)
+        s = helpers.make_system(cfg, str(tmp_path), sph_system.FP32, noise=1e-2,
+                                name=f"m{len(outs)}")
+        helpers.perturb(s)
+        s.prepare()
+        s.step(12)
+        s.synchronize()
+        outs.append([s.get(n) for n in names])
+        s.close()
+    for ref in outs[1:]:
+        for n, a, b in zip(names, outs[0], ref):
+            np.testing.assert_array_equal(a, b, err_msg=n)
