@@ -748,10 +748,12 @@ __device__ __forceinline__ void pair_sep(const Box &X, double xi, double yi, dou
 // The parameters the pair body reads, copied out of the parameter block once
 // per thread; read through the parameter pointer inside the neighbour loop
 // they were re-loaded for every neighbour.
-// Pair records with fixed-point positions: the FP32 form in 2D (whose record
-// then fits 32 bytes).  The others carry double positions.
+// Pair records with fixed-point positions: the FP32 form (the 2D record then
+// fits 32 bytes; the separation of a pair is three integer differences, with
+// the minimum image for free, instead of double arithmetic).  The FP64 form
+// carries double positions.
 template <class R, int DIM>
-constexpr bool kFixRec = sizeof(R) == 4 && DIM == 2;
+constexpr bool kFixRec = sizeof(R) == 4;
 
 // (With fixed-point records only the fixed-point scale stays live; the box and
 // the fixed-point origins are re-derived from P in the rare wall / colloid
@@ -1141,7 +1143,7 @@ struct NList {
 
 // The stored record: the fields the dimension uses, packed so that a neighbour
 // costs as few 16-byte chunks as possible (gathered as whole chunks):
-//   FP32 3D  [x y] [z vx vy] [vz w qxx qxy] [qyy qxz qyz qzz]        64 B
+//   FP32 3D  [X Y Z w] [vx vy vz qxx] [qxy qyy qxz qyz] [qzz]        64 B
 //   FP32 2D  [X Y w vx] [vy qxx qxy qyy]                              32 B
 //   FP64 3D  [x y] [z vx] [vy vz] [w qxx] [qxy qyy] [qxz qyz] [qzz]  128 B
 //   FP64 2D  [x y] [vx vy] [w qxx] [qxy qyy]                          64 B
@@ -1149,8 +1151,8 @@ template <class R, int DIM>
 struct PairRec;
 template <>
 struct alignas(64) PairRec<float, 3> {
-  double x[3];
-  float v[3], w, q[6];
+  unsigned X[3];
+  float w, v[3], q[6];
 };
 template <>
 struct alignas(32) PairRec<float, 2> {
