@@ -242,14 +242,17 @@ def time_sweep(n_sims, steps, seq_steps, tmpdir, seed=4):
     }
 
 
-def time_pipeline(n_train, n_hold, n_sweep, steps, n_range=(48, 48), seed=21, fit_steps=6000,
+def time_pipeline(n_train, n_hold, n_sweep, steps, n_range=(32, 160), seed=21, fit_steps=2000,
                   lr=1e-3, log=None):
     """The learned sweep: SPH runs train the MLP ensemble; a new sweep is then
     answered by the ensemble, with the points whose spread of the wall stress
     exceeds 5 % re-run by batched SPH.  Compared with running the whole new
-    sweep with batched SPH: wall time and wall-stress error.  The lattice is
-    fixed (n x n) so that the outputs are functions of the four inputs; the
-    physical parameters are drawn as in the uneven-size sweep."""
+    sweep with batched SPH: wall time and wall-stress error.  Points are drawn
+    as in the uneven-size sweep (sides n_range, 10,000 steps by default, the
+    observables averaged over the second half).  2000 Adam steps: the members
+    fitted longer agree less between the training points (their spread
+    exceeded 5 % for about a quarter of a sweep at 6000 steps, a twentieth at
+    2000), at the same error."""
     train = sweep.resolve(sweep.sample_points(n_train + n_hold, seed=seed, n_range=n_range))
     Y, tinfo = sweep.run_points(train, steps, log=log)
     X = surrogate.feature_matrix(train)
@@ -300,7 +303,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=200)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--pipeline-points", type=int, default=1000)
-    ap.add_argument("--fit-steps", type=int, default=6000, help="Adam steps of the surrogate")
+    ap.add_argument("--pipeline-steps", type=int, default=10000, help="SPH steps per sweep point")
+    ap.add_argument("--fit-steps", type=int, default=2000, help="Adam steps of the surrogate")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the benchmark runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
@@ -323,7 +327,8 @@ def main():
                   f"at a time", file=sys.stderr, flush=True)
         pipe = None
         if not a.no_pipeline:
-            pipe = time_pipeline(1800, 200, a.pipeline_points, 800, fit_steps=a.fit_steps)
+            pipe = time_pipeline(1800, 200, a.pipeline_points, a.pipeline_steps,
+                                 fit_steps=a.fit_steps)
             print(f"# pipeline: x{pipe['speedup']:.1f} vs plain batched, "
                   f"{pipe['fallback_points']} of {a.pipeline_points} re-run, median wall-stress "
                   f"error {pipe['wall_stress_rel_err_median']:.3%}", file=sys.stderr, flush=True)

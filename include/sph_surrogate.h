@@ -30,8 +30,8 @@ extern "C" {
 
 typedef struct sph_mlp *sph_mlp_handle;
 
-/* E members (1..64), He-uniform weights from `seed`, zero biases.  NULL on a
- * bad argument or when no GPU is available. */
+/* E members (1..64), weights and biases uniform in +-1/sqrt(fan_in) from
+ * `seed` (padding zero).  NULL on a bad argument or when no GPU is available. */
 sph_mlp_handle sph_mlp_create(int members, uint64_t seed);
 void sph_mlp_destroy(sph_mlp_handle h);
 const char *sph_mlp_last_error(sph_mlp_handle h);

@@ -468,7 +468,11 @@ sph_mlp_handle sph_mlp_create(int members, uint64_t seed) {
     delete h;
     return nullptr;
   }
-  // He-uniform weights (bound sqrt(6 / fan_in)), zero biases, per member.
+  // Weights and biases uniform in +-1/sqrt(fan_in) per member (the common
+  // default of linear layers, e.g. torch.nn.Linear).  A wider start (He-uniform,
+  // +-sqrt(6 / fan_in)) left the members of the wall-stress fit disagreeing
+  // by more than 5 % on about a third of a 1000-point sweep after 2000 Adam
+  // steps, against a twentieth with this one, at the same held-out error.
   std::mt19937_64 rng(seed);
   std::vector<float> w(n, 0.0f);
   const Layout &L = h->L;
@@ -476,11 +480,13 @@ sph_mlp_handle sph_mlp_create(int members, uint64_t seed) {
             real_cols[4] = {kIn, kHid, kHid, kHid};
   for (int e = 0; e < members; ++e)
     for (int l = 0; l < 4; ++l) {
-      const float bound = std::sqrt(6.0f / fan_in[l]) * (l == 3 ? 0.1f : 1.0f);
+      const float bound = 1.0f / std::sqrt((float)fan_in[l]);
       std::uniform_real_distribution<float> U(-bound, bound);
-      for (int r = 0; r < real_rows[l]; ++r)
+      for (int r = 0; r < real_rows[l]; ++r) {
         for (int c = 0; c < real_cols[l]; ++c)
           w[(size_t)e * L.per_member + L.w[l] + (size_t)r * L.cols[l] + c] = U(rng);
+        w[(size_t)e * L.per_member + L.b[l] + r] = U(rng);
+      }
     }
   if (sph_mlp_set_params(h, w.data(), (int64_t)n) != SPH_OK) {
     delete h;

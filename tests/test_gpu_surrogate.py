@@ -39,7 +39,17 @@ def test_layout_matches_oracle():
     W1 = p[:, w_off[0]:w_off[0] + 256 * 64].reshape(2, 256, 64)
     W4 = p[:, w_off[3]:w_off[3] + 64 * 256].reshape(2, 64, 256)
     assert np.all(W1[:, :, 4:] == 0) and np.all(W4[:, 3:] == 0)
-    assert np.all(p[:, b_off[0]:] == 0)            # zero biases
+    # weights and biases of the real units uniform in +-1/sqrt(fan_in), padding zero
+    fan_in = [4, 256, 256, 256]
+    rows = [256, 256, 256, 3]
+    for l in range(4):
+        W = p[:, w_off[l]:w_off[l] + mo.ROWS[l] * mo.COLS[l]].reshape(2, mo.ROWS[l], mo.COLS[l])
+        bias = p[:, b_off[l]:b_off[l] + mo.ROWS[l]]
+        bound = 1.0 / np.sqrt(fan_in[l])
+        real = W[:, :rows[l], :fan_in[l]]
+        assert np.all(np.abs(real) <= bound) and np.abs(real).max() > 0.9 * bound
+        assert np.all(np.abs(bias[:, :rows[l]]) <= bound)
+        assert np.all(bias[:, rows[l]:] == 0)
     assert not np.array_equal(p[0], p[1])          # members differ
 
 
