@@ -997,12 +997,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // assert) raises its simulation's flag and is filed under the nearest cell, so
 // the sorted order stays a permutation within each simulation and the other
 // simulations of a batch are unaffected.
+// With `rebuild` (a Verlet step) a simulation that keeps its order is not
+// hashed: its particle count goes to its first cell, so that the scan still
+// gives every rebuilt simulation's cells their places (its own cell starts are
+// not read until its next rebuild).
 __global__ __launch_bounds__(kBlock) void k_hash(SimTable T, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z, int *cell, int *slot,
-                                                 int *count, int *flag, const int *any) {
+                                                 int *count, int *flag, const int *rebuild,
+                                                 const int *any) {
   if (any && !*any) return;  // no simulation rebuilds this step
   const int b = blockIdx.x, sim = T.blk_sim[b];
+  if (rebuild && (!rebuild[sim] || T.halt[sim])) {
+    if (threadIdx.x == 0 && T.blk_first[b] == T.pbeg[sim])
+      atomicAdd(&count[T.cell_off[sim]], T.pbeg[sim + 1] - T.pbeg[sim]);
+    return;
+  }
   const sph_params_t &P = T.P[sim];
   const int i = T.blk_first[b] + threadIdx.x;
   int c = -2;  // lanes past the simulation's last particle
@@ -1789,7 +1799,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   unsigned sw[4] = {0u, 0u, 0u, 0u};
   uint4 *__restrict__ sout = slots ? VL.slot + i : nullptr;
   // The cut-off test of the density pass (float positions in the FP32 form),
-  // four candidates' loads in flight at a time.  Entries are indices within
+  // eight candidates' loads in flight at a time.  Entries are indices within
   // the simulation (k - pbeg), the offsets the passes' buffer loads take.
   CandSelf<R, DIM> me;
   me.init(S, i, P);
@@ -1813,12 +1823,20 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     me.shift(sx, sy, sz);
     int k = k0;
 #pragma unroll 1
-    for (; k + 4 <= k1; k += 4) {
+    for (; k + 8 <= k1; k += 8) {
+      Cand<R, DIM> c[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) c[u].load(S, k + u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) test(k + u, c[u]);
+    }
+    if (k + 4 <= k1) {
       Cand<R, DIM> c[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) c[u].load(S, k + u);
 #pragma unroll
       for (int u = 0; u < 4; ++u) test(k + u, c[u]);
+      k += 4;
     }
 #pragma unroll 1
     for (; k < k1; ++k) {
@@ -3077,11 +3095,15 @@ class DeviceImpl final : public Device {
     vcap_ = ((int)(1.5 * vmean) + 64 + 7) / 8 * 8;
     if (const char *e = std::getenv("SPH_VERLET_CAPACITY"))
       vcap_ = std::max(8, (std::atoi(e) + 7) / 8 * 8);
-    // Staged FP32 density walk (SPH_DENSITY_STAGE=0 turns it off; SPH_STAGE_CAP
-    // lowers the positions a window may hold, so that more tiles take the
-    // unstaged walk; results are the same either way)
-    stage_ = sizeof(R) == 4 && verlet_ok_;
-    if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_ = stage_ && std::atoi(e) != 0;
+    // Staged FP32 density walk, 3D (SPH_DENSITY_STAGE=0 turns it off, =2 turns
+    // it on in 2D too; SPH_STAGE_CAP lowers the positions a window may hold,
+    // so that more tiles take the unstaged walk; results are the same either
+    // way).  In 2D (about 30 entries per particle) the window copy and the
+    // slots of the frequent rebuilds of a batch cost more than the L1 gathers
+    // they replace: a 55-simulation sweep batch stepped 10 % slower.
+    int stage_mode = 1;
+    if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_mode = std::atoi(e);
+    stage_ = sizeof(R) == 4 && verlet_ok_ && stage_mode != 0 && (P_.dim == 3 || stage_mode == 2);
     stage_cap_ = kStageCap;
     if (const char *e = std::getenv("SPH_STAGE_CAP"))
       stage_cap_ = std::min(kStageCap, std::max(0, std::atoi(e)));
@@ -3312,7 +3334,7 @@ class DeviceImpl final : public Device {
     Particles<R> S = parts(cur_), T = parts(cur_ ^ 1);
     int *any = gate ? any_.p : nullptr;
     k_hash<<<nblk_, block_, 0, stream_>>>(table(), S.x, S.y, S.z, cell_.p, slot_.p, count_.p,
-                                          flag_.p, any);
+                                          flag_.p, gate, any);
     const int ntiles = (ncells_ + kScanTile - 1) / kScanTile;
     k_scan_tiles<<<ntiles, 1024, 0, stream_>>>(count_.p, cs_.p, tsum_.p, ncells_, any);
     if (ntiles > scan_sum_tiles_) {

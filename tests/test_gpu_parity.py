@@ -386,8 +386,9 @@ def test_observables(case, tmp_path):
 @pytest.mark.parametrize("case", ["colloids3d", "sine3d", "couette2d"])
 def test_staged_density_walk_is_the_unstaged_one(case, tmp_path, monkeypatch):
     """The FP32 density walk reads neighbour positions from a per-tile LDS
-    window (SPH_DENSITY_STAGE=0 reads them from memory; SPH_STAGE_CAP=400
-    leaves the tiles with larger windows to the unstaged walk).  The same
+    window (SPH_DENSITY_STAGE=2: in 2D too; =0 reads them from memory;
+    SPH_STAGE_CAP=400 leaves the tiles with larger windows to the unstaged
+    walk).  The same
     positions in the same order: bitwise the same trajectory, with list
     rebuilds (a skin of 0.01 rcut) inside the run."""
     monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
@@ -396,7 +397,8 @@ def test_staged_density_walk_is_the_unstaged_one(case, tmp_path, monkeypatch):
            "sine3d": lambda: helpers.cfg_sine(n=24)}[case]()
     names = ("x", "y", "vx", "vy", "dens", "fx", "fy", "cxy")
     outs = []
-    for mode in ({}, {"SPH_DENSITY_STAGE": "0"}, {"SPH_STAGE_CAP": "400"}):
+    for mode in ({"SPH_DENSITY_STAGE": "2"}, {"SPH_DENSITY_STAGE": "0"},
+                 {"SPH_DENSITY_STAGE": "2", "SPH_STAGE_CAP": "400"}):
         for k in ("SPH_DENSITY_STAGE", "SPH_STAGE_CAP"):
             monkeypatch.delenv(k, raising=False)
         for k, v in mode.items():
