@@ -8,7 +8,9 @@ gradient/dc-dt + forces + wall/colloid forces + VV2, all on the GPU, state
 resident in HBM.  `value` is the FP32 form (pair arithmetic in float,
 positions and reductions in double); the FP64 form is reported beside it.
 
-Also measured (the "sweep" object): the batched parameter sweep - 256
+Also measured: the 64^3 periodic sine-forced flow in both forms (the
+"c2_periodic_64cubed" object), and (the "sweep" object) the batched parameter
+sweep - 256
 independent 2D wall + colloid simulations of uneven size (n per side 32-160,
 V, tau, colloid count, field frequency and strength drawn from a seed), FP32
 form, all in one batch whose step is one launch per phase, captured as a
@@ -79,7 +81,7 @@ TRAFFIC_FILE = "profiles/traffic_walls_colloids_1M_fp32.json"
 PAIR_PROFILE = "profiles/pair_passes_walls_colloids_1M_fp32.md"
 # VALU instructions x 4 cycles / SIMD busy time of the FP32 pair passes on the
 # 1M-particle workload (SQ_INSTS_VALU, SQ_BUSY_CYCLES; see PAIR_PROFILE)
-VALU_BUSY = {"k_pairs": 0.70, "k_density": 0.53}
+VALU_BUSY = {"k_pairs": 0.77, "k_density": 0.89}
 
 
 def measured_traffic(kernel, N):
@@ -194,6 +196,34 @@ def cpu_baseline(n, n_colloids, tmpdir, steps=5, warmup=10, repeats=3, with_smal
     out["nproc"] = os.cpu_count()
     out["isa"] = "x86-64-v4 (AVX-512)" if lib == oracle.LIB_V4_PATH else "x86-64-v3 (AVX2)"
     out["warmup"], out["repeats"] = warmup, repeats
+    return out
+
+
+def time_c2(steps, warmup, tmpdir, seed=5):
+    """The 64^3 periodic sine-forced Oldroyd-B flow (262,144 particles, tau 0.5,
+    dt 5e-5, force amplitude 1 along x varying in y): particle-steps/s of each
+    precision form, with rebuilds counted as in the main workload."""
+    out = {"workload": "3D periodic sine-forced flow, 64^3 lattice, tau 0.5, dt 5e-5"}
+    for prec, name in ((sph_system.FP32, "fp32"), (sph_system.FP64, "fp64")):
+        path = os.path.join(tmpdir, f"input_c2_{name}")
+        inputs.write_input(path, inputs.sine_3d(n=64))
+        s = sph_system.System(prec)
+        s.read_input(path)
+        s.initialize()
+        inputs.seeded_velocity(s, 1e-3, seed)
+        s.prepare()
+        s.step(warmup)
+        s.synchronize()
+        c0 = s.counters()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        t = time.perf_counter() - t0
+        c1 = s.counters()
+        out[name] = {"value": s.N * steps / t, "ms_per_step": 1e3 * t / steps,
+                     "rebuilds_in_window": c1["rebuilds"] - c0["rebuilds"]}
+        out["particles"] = s.N
+        s.close()
     return out
 
 
@@ -319,6 +349,9 @@ def main():
             r64 = time_gpu(sph_system.FP64, a.n, a.colloids, a.steps, a.warmup, tmpdir)
             print(f"# fp64: {a.steps} steps in {r64['elapsed']:.3f}s", file=sys.stderr,
                   flush=True)
+        c2 = time_c2(max(a.steps, 100), a.warmup, tmpdir)
+        print(f"# c2 64^3: fp32 {c2['fp32']['ms_per_step']:.3f} ms/step, fp64 "
+              f"{c2['fp64']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
         sw = None
         if not a.no_sweep:
             sw = time_sweep(a.sweep_sims, a.sweep_steps, 20, tmpdir)
@@ -418,6 +451,7 @@ def main():
             "note": "peak counts packed FP32 (v_pk_fma_f32); this build is compiled without "
                     "packed FP32, whose ceiling is peak_unpacked",
         },
+        "c2_periodic_64cubed": c2,
         "sweep": sw,
         "sweep_pipeline": pipe,
         "cpu_baseline": None if cpu is None else {
