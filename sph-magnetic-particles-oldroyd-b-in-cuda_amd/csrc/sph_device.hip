@@ -401,29 +401,28 @@ __device__ __forceinline__ R sph_W(R r, R rcut_inv, R cw) {
   return cw * t;
 }
 
-// The spline polynomial of kernel_W without cw, from q3 = 3 r / rcut:
-// (a^5 - 6 b^5) + 15 c^5 with a = 3 - q3, b = 2 - q3, c = 1 - q3 (terms dropped
-// past their range), so that a sum of W is cw times a sum of these.
+// The spline polynomial of kernel_W without cw, from q3 = 3 r / rcut, so that a
+// sum of W is cw times a sum of these; branch-free: a^5 - 6 b^5 + 15 c^5 with
+// a = 3 - q3, b = max(2 - q3, 0), c = max(1 - q3, 0) (a term past its range is
+// an exact 0).
 template <class R>
 __device__ __forceinline__ R sph_W_poly(R q3) {
-  R a = R(3) - q3, a2 = a * a, t = a2 * a2 * a;
-  R b = R(2) - q3, b2 = b * b, b5 = R(6) * (b2 * b2 * b);
-  R c = R(1) - q3, c2 = c * c, c5 = R(15) * (c2 * c2 * c);
-  if (q3 < R(2)) t = t - b5;
-  if (q3 < R(1)) t = t + c5;
-  return t;
+  const R a = R(3) - q3, a2 = a * a;
+  const R b = fmax(R(2) - q3, R(0)), b2 = b * b;
+  const R c = fmax(R(1) - q3, R(0)), c2 = c * c;
+  return fma(R(15), c2 * c2 * c, fma(R(-6), b2 * b2 * b, a2 * a2 * a));
 }
 
-// kernel_gradW (src/kernel_gradW.cu:13-53): c_gradw*((a^4 - 6 b^4) + 15 c^4).
+// kernel_gradW (src/kernel_gradW.cu:13-53) is c_gradw*((a^4 - 6 b^4) + 15 c^4)
+// piecewise in q3 = 3 r / rcut.  Its polynomial without c_gradw (0 <= q3 < 3),
+// branch-free: a^4 - 6 b^4 + 15 c^4 with a = 3 - q3, b = max(2 - q3, 0),
+// c = max(1 - q3, 0) - a term past its range is an exact 0.
 template <class R>
-__device__ __forceinline__ R sph_gradW(R r, R rcut_inv, R c_gradw) {
-  R q3 = R(3) * (r * rcut_inv);
-  R a = R(3) - q3, a2 = a * a, t = a2 * a2;
-  R b = R(2) - q3, b2 = b * b, b4 = R(6) * (b2 * b2);
-  R c = R(1) - q3, c2 = c * c, c4 = R(15) * (c2 * c2);
-  if (q3 < R(2)) t = t - b4;
-  if (q3 < R(1)) t = t + c4;
-  return c_gradw * t;
+__device__ __forceinline__ R sph_gradW_poly(R q3) {
+  const R a = R(3) - q3, a2 = a * a;
+  const R b = fmax(R(2) - q3, R(0)), b2 = b * b;
+  const R c = fmax(R(1) - q3, R(0)), c2 = c * c;
+  return fma(R(15), c2 * c2, fma(R(-6), b2 * b2, a2 * a2));
 }
 
 // Unclamped cell coordinates of a particle, as the pair kernels of the
@@ -755,6 +754,7 @@ __device__ __forceinline__ void pair_sep(const Box &X, double xi, double yi, dou
 template <class R, int DIM>
 constexpr bool kFixRec = sizeof(R) == 4;
 
+
 // (With fixed-point records only the fixed-point scale stays live; the box and
 // the fixed-point origins are re-derived from P in the rare wall / colloid
 // branches, which keeps the loop's scalar registers for the hot values.)
@@ -764,11 +764,12 @@ struct PairK {
   Box X;       // double positions: the minimum image of every pair
   float u[3];  // fixed-point positions: span / 2^32
   double y_bottom, y_top;
-  R rcut_inv, c_gradw, bmax, nkT, a, b, V_bottom, V_top, coll_R;
+  // q3s = 3 / rcut, ncg = -c_gradw, na = -a, nb = -b
+  R q3s, ncg, bmax, nkT, na, nb, V_bottom, V_top, coll_R;
   __device__ explicit PairK(const sph_params_t &P_)
       : P(&P_), X(kFixRec<R, DIM> ? Box{} : box_of(P_)), y_bottom(P_.y_bottom),
-        y_top(P_.y_top), rcut_inv((R)P_.rcut_inv), c_gradw((R)P_.c_gradw),
-        bmax((R)P_.beta_max), nkT((R)P_.nkT), a((R)P_.a), b((R)P_.b),
+        y_top(P_.y_top), q3s(R(3) * (R)P_.rcut_inv), ncg(-(R)P_.c_gradw),
+        bmax((R)P_.beta_max), nkT((R)P_.nkT), na(-(R)P_.a), nb(-(R)P_.b),
         V_bottom((R)P_.V_bottom), V_top((R)P_.V_top), coll_R((R)P_.coll_R) {
     if constexpr (kFixRec<R, DIM>) {
       const Fix f = fix_of(P_);
@@ -822,7 +823,6 @@ struct PairSelf {
   int t;
   R v[3];
   R dinv, p;
-  R pd[6];
   // coordinate a as a double (fixed-point values mapped back)
   template <int DIM>
   __device__ __forceinline__ double pos(const PairK<R, DIM> &K, int a) const {
@@ -1913,24 +1913,40 @@ __device__ __forceinline__ void ext_force(const sph_params_t &P, R m, double y, 
   }
 }
 
-// Per-particle accumulators of the fused pair pass.
+// Per-particle accumulators of the fused pair pass, as sums of ge = -gradW(r) e
+// (e the unit vector from j to i):
+//   velocity gradient  (src/kernel_vel_gradients.cu:632-640)
+//     gradv_ab = -(1/d_i) sum gradW e_a v_b = (1/d_i) g_ab,  g_ab = sum ge_a v_b;
+//   reversible force   (src/kernel_forces.cu:429-433)
+//     -sum gradW (Pi_i/d_i^2 + Pi_j/d_j^2).e = (Pi_i/d_i^2) s + sum (Pi_j/d_j^2) ge,
+//     s = sum ge: particle i's own stress enters once per particle instead of once per pair;
+//   irreversible force (src/kernel_forces.cu:436-440) summed into f per pair.
+// pair_finish scales g and adds (Pi_i/d_i^2) s.
 template <class R>
 struct PairAcc {
-  R g[3][3];  // velocity gradient sum
-  R f[3];     // force sum
+  R g[3][3];  // sum of ge_a v_b
+  R s[3];     // sum of ge over the force pairs
+  R f[3];     // force sum without the (Pi_i/d_i^2) s part
 };
 
 
 // One neighbour j of i: velocity-gradient and force contributions with the
 // reference's pair selection, Morris velocities and Pi/d^2 substitutions
 // (src/kernel_vel_gradients.cu:123-640, src/kernel_forces.cu:154-850).
+// A fluid - fluid pair (j's w = 1/d > 0) is told apart by one test; j's type
+// and the pair selection are worked out only for the others.
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F, class CV>
 __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<R> &rj,
                                           const CV &B, const PairSelf<R> &me,
                                           PairAcc<R> &acc) {
-  const int tj = rj.w < R(0) ? (int)(-rj.w) : 0;
-  const bool on_gv = DO_GV && (me.t == 0 || tj != me.t);
-  const bool on_f = DO_F && (me.t == 0 || tj == 0);
+  const bool ff = me.t == 0 && rj.w > R(0);
+  int tj = 0;
+  bool on_gv = DO_GV, on_f = DO_F;
+  if (!ff) {
+    tj = rj.w > R(0) ? 0 : (int)(-rj.w);
+    on_gv = DO_GV && (me.t == 0 || tj != me.t);
+    on_f = DO_F && (me.t == 0 || tj == 0);
+  }
   if (!on_gv && !on_f) return;
   R r[3];
   if constexpr (kFixRec<R, DIM>) {
@@ -1943,11 +1959,15 @@ __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<
   const R r2 = DIM == 3 ? r[0] * r[0] + r[1] * r[1] + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
   R rr, ri;
   pair_root(r2, rr, ri);
-  const R e[3] = {r[0] * ri, r[1] * ri, DIM == 3 ? r[2] * ri : R(0)};
-  const R gw = sph_gradW<R>(rr, K.rcut_inv, K.c_gradw);
+  // ge = -gradW(r) e = gr r, gradW = c_gradw * poly(3 r / rcut) (src/kernel_gradW.cu:13-53)
+  const R gr = sph_gradW_poly<R>(rr * K.q3s) * (ri * K.ncg);
+  const R ge[3] = {gr * r[0], gr * r[1], DIM == 3 ? gr * r[2] : R(0)};
   const R vj[3] = {rj.v[0], rj.v[1], rj.v[2]};
   R v[3];
-  if constexpr (kFixRec<R, DIM>) {
+  if (ff) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = me.v[a] - vj[a];
+  } else if constexpr (kFixRec<R, DIM>) {
     // (the fixed-point coordinates mapped back; used only by wall and colloid pairs)
     morris_vij<R, DIM, WALL>(K, B, me.t, tj, me.pos(K, 0), me.pos(K, 1),
                              DIM == 3 ? me.pos(K, 2) : 0.0, pos_of(K, rj, 0), pos_of(K, rj, 1),
@@ -1960,68 +1980,59 @@ __device__ __forceinline__ void pair_body(const PairK<R, DIM> &K, const PairVal<
                              r, v);
   }
   if (on_gv) {
-    // gradv_ab += -(1/d_i) gradW e_a v_b  (src/kernel_vel_gradients.cu:632-640)
-    const R coef = -me.dinv * gw;
 #pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      const R ce = coef * e[a];
+    for (int a = 0; a < DIM; ++a)
 #pragma unroll
-      for (int b = 0; b < DIM; ++b) acc.g[a][b] += ce * v[b];
-    }
+      for (int b = 0; b < DIM; ++b) acc.g[a][b] = fma(ge[a], v[b], acc.g[a][b]);
   }
   if (on_f) {
-    // Pi/d_i^2 + Pj/d_j^2, formed in each case's own registers (a common
-    // Pi, Pj pair made the compiler copy both into place for every pair)
-    R Ps[6], diinv, djinv;
-    if (me.t == tj) {  // fluid - fluid
-      diinv = me.dinv;
-      djinv = rj.w;
+    // Q = Pi_j/d_j^2 as the pair's case has it, dd = 1/(d_i d_j)
+    // (src/kernel_forces.cu:196-233, 319-336, 429-440).  Fluid - fluid: the
+    // record's Pi_j/d_j^2 as it is.
+    R Q[6], dd;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) Ps[q] = me.pd[q] + rj.q[q];
+    for (int q = 0; q < 6; ++q) Q[q] = rj.q[q];
+    if (ff) {
+      dd = me.dinv * rj.w;
     } else if (me.t == 0) {  // wall/colloid j takes i's density and pressure, keeps its c
-      diinv = me.dinv;
-      djinv = me.dinv;
-      const R nkT = K.nkT, d2 = djinv * djinv;
-      R Pj[6];
-      Pj[0] = (me.p - nkT * rj.q[0]) * d2;
-      Pj[1] = (-nkT * rj.q[1]) * d2;
-      Pj[2] = (me.p - nkT * rj.q[2]) * d2;
-      Pj[3] = DIM == 3 ? (-nkT * rj.q[3]) * d2 : R(0);
-      Pj[4] = DIM == 3 ? (-nkT * rj.q[4]) * d2 : R(0);
-      Pj[5] = DIM == 3 ? (me.p - nkT * rj.q[5]) * d2 : R(0);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) Ps[q] = me.pd[q] + Pj[q];
+      const R nkT = K.nkT, d2 = me.dinv * me.dinv;
+      Q[0] = (me.p - nkT * Q[0]) * d2;
+      Q[1] = (-nkT * Q[1]) * d2;
+      Q[2] = (me.p - nkT * Q[2]) * d2;
+      if (DIM == 3) {
+        Q[3] = (-nkT * Q[3]) * d2;
+        Q[4] = (-nkT * Q[4]) * d2;
+        Q[5] = (me.p - nkT * Q[5]) * d2;
+      }
+      dd = d2;
     } else {  // wall/colloid i: both sides use fluid j's d, p and c
-      djinv = rj.w;
-      diinv = djinv;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) Ps[q] = rj.q[q] + rj.q[q];
+      for (int q = 0; q < 6; ++q) Q[q] = Q[q] + Q[q];
+      dd = rj.w * rj.w;
     }
-    const R ev = DIM == 3 ? e[0] * v[0] + e[1] * v[1] + e[2] * v[2] : e[0] * v[0] + e[1] * v[1];
-    const R axx = -gw * Ps[0], axy = -gw * Ps[1];
-    const R ayy = -gw * Ps[2];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) acc.s[a] += ge[a];
     if (DIM == 3) {
-      const R axz = -gw * Ps[3], ayz = -gw * Ps[4];
-      const R azz = -gw * Ps[5];
-      acc.f[0] += axx * e[0] + axy * e[1] + axz * e[2];
-      acc.f[1] += axy * e[0] + ayy * e[1] + ayz * e[2];
-      acc.f[2] += axz * e[0] + ayz * e[1] + azz * e[2];
+      acc.f[0] = fma(Q[0], ge[0], fma(Q[1], ge[1], fma(Q[3], ge[2], acc.f[0])));
+      acc.f[1] = fma(Q[1], ge[0], fma(Q[2], ge[1], fma(Q[4], ge[2], acc.f[1])));
+      acc.f[2] = fma(Q[3], ge[0], fma(Q[4], ge[1], fma(Q[5], ge[2], acc.f[2])));
     } else {
-      acc.f[0] += axx * e[0] + axy * e[1];
-      acc.f[1] += axy * e[0] + ayy * e[1];
+      acc.f[0] = fma(Q[0], ge[0], fma(Q[1], ge[1], acc.f[0]));
+      acc.f[1] = fma(Q[1], ge[0], fma(Q[2], ge[1], acc.f[1]));
     }
-    const R aux = gw * diinv * djinv * ri;
-    const R a1 = K.a * aux, a2 = K.b * aux * ev;
-    acc.f[0] += a1 * v[0] + a2 * e[0];
-    acc.f[1] += a1 * v[1] + a2 * e[1];
-    if (DIM == 3) acc.f[2] += a1 * v[2] + a2 * e[2];
+    // gradW/(d_i d_j r) (a v + b (e.v) e) = -(gr dd)(a v + b (r.v)/r^2 r)
+    const R rv = DIM == 3 ? r[0] * v[0] + r[1] * v[1] + r[2] * v[2] : r[0] * v[0] + r[1] * v[1];
+    const R h = gr * dd;
+    const R ca = h * K.na, cb = (h * K.nb) * (rv * (ri * ri));
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) acc.f[a] = fma(ca, v[a], fma(cb, r[a], acc.f[a]));
   }
 }
 
 template <class R, int DIM>
 __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles<R> &S,
-                                          const Derived<R> &D, const PairRec<R, DIM> *rec,
-                                          int i, bool forces, PairSelf<R> &me) {
+                                          const Derived<R> &D, int i, bool forces,
+                                          PairSelf<R> &me) {
   if constexpr (kFixRec<R, DIM>) {
     const uint4 X = S.pq[i];
     me.X[0] = X.x;
@@ -2038,31 +2049,57 @@ __device__ __forceinline__ void load_self(const sph_params_t &P, const Particles
   me.v[2] = DIM == 3 ? S.v[2][i] : R(0);
   me.dinv = D.dinv[i];
   me.p = forces ? D.press[i] : R(0);
-  // Pi/d^2 of a fluid particle (only fluid particles use their own)
-  const bool own = forces && me.t == 0;
+}
+
+// Adds (Pi_i/d_i^2) s to the force sum and clears s (fluid i; a wall or colloid
+// particle's force holds its partners' stresses only).
+template <class R, int DIM>
+__device__ __forceinline__ void add_self_stress(const PairRec<R, DIM> *rec, int i,
+                                                const PairSelf<R> &me, PairAcc<R> &acc) {
+  if (me.t == 0) {
+    R q[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) me.pd[q] = own && (DIM == 3 || q < 3) ? rec[i].q[q] : R(0);
+    for (int k = 0; k < 6; ++k) q[k] = DIM == 3 || k < 3 ? rec[i].q[k] : R(0);
+    if (DIM == 3) {
+      acc.f[0] = fma(q[0], acc.s[0], fma(q[1], acc.s[1], fma(q[3], acc.s[2], acc.f[0])));
+      acc.f[1] = fma(q[1], acc.s[0], fma(q[2], acc.s[1], fma(q[4], acc.s[2], acc.f[1])));
+      acc.f[2] = fma(q[3], acc.s[0], fma(q[4], acc.s[1], fma(q[5], acc.s[2], acc.f[2])));
+    } else {
+      acc.f[0] = fma(q[0], acc.s[0], fma(q[1], acc.s[1], acc.f[0]));
+      acc.f[1] = fma(q[1], acc.s[0], fma(q[2], acc.s[1], acc.f[1]));
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) acc.s[a] = R(0);
 }
 
 // Writes the results of particle i: force (+ external force), gradient tensor
 // (store_gv), dc/dt of fluid particles (write_dcdt; src/kernel_dcdt.cu:76-102).
 template <class R, int DIM, bool DO_GV, bool DO_F>
 __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particles<R> &S,
-                                            const Derived<R> &D, int i, const PairSelf<R> &me,
-                                            PairAcc<R> &acc, int store_gv, int write_dcdt,
-                                            int with_ext, int kick) {
+                                            const Derived<R> &D,
+                                            const PairRec<R, DIM> *rec, int i,
+                                            const PairSelf<R> &me, PairAcc<R> &acc,
+                                            int store_gv, int write_dcdt, int with_ext,
+                                            int kick) {
   if (DO_F) {
+    add_self_stress<R, DIM>(rec, i, me, acc);
     if (with_ext && me.t == 0) ext_force<R, DIM>(P, S.mass[i], S.y[i], acc.f);
     D.f[0][i] = acc.f[0];
     D.f[1][i] = acc.f[1];
     D.f[2][i] = DIM == 3 ? acc.f[2] : R(0);
   }
   if (!DO_GV) return;
+  R g[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) g[a][b] = me.dinv * acc.g[a][b];
   if (store_gv) {
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
 #pragma unroll
-      for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = acc.g[a][b];
+      for (int b = 0; b < DIM; ++b) D.gv[a * 3 + b][i] = g[a][b];
   }
   if (!write_dcdt) return;
   if (me.t != 0) {
@@ -2070,7 +2107,6 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
     for (int k = 0; k < (DIM == 3 ? 6 : 3); ++k) D.dcdt[k][i] = R(0);
     return;
   }
-  const R(&g)[3][3] = acc.g;
   const R tinv = R(1) / (R)P.tau;
   R c[6], dc[6];
 #pragma unroll
@@ -2167,11 +2203,12 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   const int i = live ? i0 : iend - 1;
   const int N = T.ntotal;  // list stride
   PairSelf<R> me;
-  load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
+  load_self<R, DIM>(P, S, D, i, DO_F, me);
   PairAcc<R> acc;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     acc.f[a] = R(0);
+    acc.s[a] = R(0);
 #pragma unroll
     for (int b = 0; b < 3; ++b) acc.g[a][b] = R(0);
   }
@@ -2314,8 +2351,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     }
     if (!live) return;
     if (vok) {
-      pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
-                                     kick);
+      pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, rec, i, me, acc, store_gv, write_dcdt,
+                                       with_ext, kick);
       return;
     }
   }
@@ -2337,7 +2374,10 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
   if (total > cnt) {
     // Park the partial sums this pass computes (and only those: a gradient-only
     // pass leaves the forces of an earlier force pass alone, and a force-only
-    // pass the gradient); k_pairs_tail continues from them.
+    // pass the gradient); k_pairs_tail continues from them.  The force is
+    // parked with i's own stress term of the pairs so far, the gradient
+    // unscaled.
+    if (DO_F) add_self_stress<R, DIM>(rec, i, me, acc);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (DO_F) D.f[a][i] = acc.f[a];
@@ -2348,8 +2388,8 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     L.queue[atomicAdd(L.qlen, 1)] = i;
     return;
   }
-  pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
-                                     kick);
+  pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, rec, i, me, acc, store_gv, write_dcdt, with_ext,
+                                   kick);
 }
 
 // Particles whose neighbours did not all fit in the list: re-scan the strips
@@ -2370,11 +2410,12 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
     const Bodies B = bodies_of(Ball, T, sim);
     const int *__restrict__ cs = csg + T.cell_off[sim];
     PairSelf<R> me;
-    load_self<R, DIM>(P, S, D, rec, i, DO_F, me);
+    load_self<R, DIM>(P, S, D, i, DO_F, me);
     PairAcc<R> acc;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       acc.f[a] = DO_F ? D.f[a][i] : R(0);
+      acc.s[a] = R(0);
 #pragma unroll
       for (int b = 0; b < 3; ++b) acc.g[a][b] = DO_GV ? D.gv[a * 3 + b][i] : R(0);
     }
@@ -2401,8 +2442,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs_tail(SimTable T, Particles<R> 
         }
       }
     });
-    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, i, me, acc, store_gv, write_dcdt, with_ext,
-                                     kick);
+    pair_finish<R, DIM, DO_GV, DO_F>(P, S, D, rec, i, me, acc, store_gv, write_dcdt,
+                                     with_ext, kick);
   }
 }
 
