@@ -279,10 +279,13 @@ def time_pipeline(n_train, n_hold, n_sweep, steps, n_range=(32, 160), seed=21, f
     exceeds 5 % re-run by batched SPH.  Compared with running the whole new
     sweep with batched SPH: wall time and wall-stress error.  Points are drawn
     as in the uneven-size sweep (sides n_range, 10,000 steps by default, the
-    observables averaged over the second half).  2000 Adam steps: the members
-    fitted longer agree less between the training points (their spread
-    exceeded 5 % for about a quarter of a sweep at 6000 steps, a twentieth at
-    2000), at the same error."""
+    observables averaged over the second half).  Runs whose wall stress is not
+    that of a steady sheared flow (its samples swing by more than the
+    Newtonian wall stress: a colloid configuration gone unstable) count as
+    failed, in the training data and in the plain sweep alike
+    (`unstable_sph_runs`); left in the training data, a dozen such runs with
+    stresses of either sign at many times their scale made the members
+    disagree at about a tenth of the sweep's points.  2000 Adam steps."""
     train = sweep.resolve(sweep.sample_points(n_train + n_hold, seed=seed, n_range=n_range))
     Y, tinfo = sweep.run_points(train, steps, log=log)
     X = surrogate.feature_matrix(train)
@@ -314,6 +317,7 @@ def time_pipeline(n_train, n_hold, n_sweep, steps, n_range=(32, 160), seed=21, f
         "wall_stress_rel_err_p90": float(np.percentile(err, 90)),
         "held_out_rel_err_median": float(np.median(held_err)),
         "failed_sph_runs": int(pinfo["failed"] + tinfo["failed"]),
+        "unstable_sph_runs": int(pinfo["unstable"] + tinfo["unstable"]),
         "train_data_s": tinfo["seconds"], "fit_s": t_fit,
     }
 

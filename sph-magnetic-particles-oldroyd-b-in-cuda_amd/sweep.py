@@ -130,7 +130,17 @@ def _circ_half(angles):
     return 0.5 * math.atan2(np.sin(2 * a).mean(), np.cos(2 * a).mean())
 
 
-def _run_batch(b, g, steps, samples, autotune, targets, info, log):
+def unsteady(wall, eta, dV, L1, factor=1.0):
+    """True when the wall-stress samples `wall` of a run are not those of a
+    steady sheared flow: their standard deviation exceeds `factor` times the
+    Newtonian wall stress eta |dV| / L1 of the walls' relative speed dV (a
+    colloid configuration gone unstable swings the stress to either sign by
+    many times its scale; a steady run of the sweep fluctuates by a few per
+    cent of it)."""
+    return bool(np.std(np.asarray(wall, np.float64)) > factor * eta * abs(dV) / L1)
+
+
+def _run_batch(b, g, steps, samples, autotune, targets, info, log, unstable_factor=1.0):
     n_part = sum(p.N for p in b.params)
     t0 = time.perf_counter()
     b.prepare()
@@ -161,6 +171,11 @@ def _run_batch(b, g, steps, samples, autotune, targets, info, log):
             info["failed"] += 1
             continue
         wall = 0.5 * (o[:, j, 3] - o[:, j, 4])
+        p = b.params[j]
+        if unsteady(wall, p.eta, p.V_top - p.V_bottom, p.L[1], unstable_factor):
+            info["failed"] += 1
+            info["unstable"] += 1
+            continue
         targets[i, 0] = float(np.mean(wall))
         # how settled the average is: the two halves of the window, relative
         h = len(wall) // 2
@@ -174,17 +189,23 @@ def _run_batch(b, g, steps, samples, autotune, targets, info, log):
 
 
 def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000,
-               samples=20, autotune=False, log=None):
+               samples=20, autotune=False, log=None, unstable_factor=1.0):
     """Runs every point for `steps` steps in batches of at most `max_particles`
     particles and returns (targets [n, 3], info).  Observables are sampled
     `samples` times over the second half of the run and averaged.  A point
     whose simulation fails (a particle leaves the grid - e.g. a colloid driven
     into a wall - or its energy is not finite) gets NaN targets and is counted
-    in info["failed"]; the rest of its batch is unaffected.  info["seconds"]
+    in info["failed"]; the rest of its batch is unaffected.  So does a point
+    whose wall-stress samples are not those of a steady sheared flow: their
+    standard deviation exceeds `unstable_factor` times the Newtonian wall
+    stress eta (V_top - V_bottom) / L1 (counted in info["unstable"] as well;
+    these are runs whose colloids went unstable - the stress swings to either
+    sign by many times its scale - while no particle has left the grid yet).
+    info["seconds"]
     is the device time of the batches (upload, steps, observables), without
     the host set-up of the inputs."""
     targets = np.full((len(points), 3), np.nan)
-    info = dict(batches=0, particle_steps=0, seconds=0.0, particles=[], failed=0,
+    info = dict(batches=0, particle_steps=0, seconds=0.0, particles=[], failed=0, unstable=0,
                 settle=np.full(len(points), np.nan))
     order = sorted(range(len(points)), key=lambda i: -points[i]["n"])
     with tempfile.TemporaryDirectory() as tmpdir:
@@ -192,7 +213,7 @@ def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000
         for i in order:
             s = build_system(points[i], steps, precision, tmpdir)
             if g and total + s.N > max_particles:
-                _run_batch(b, g, steps, samples, autotune, targets, info, log)
+                _run_batch(b, g, steps, samples, autotune, targets, info, log, unstable_factor)
                 b.close()
                 b, g, total = None, [], 0
             if b is None:
@@ -202,6 +223,6 @@ def run_points(points, steps, precision=sph_system.FP32, max_particles=4_000_000
             total += s.N
             s.close()
         if g:
-            _run_batch(b, g, steps, samples, autotune, targets, info, log)
+            _run_batch(b, g, steps, samples, autotune, targets, info, log, unstable_factor)
             b.close()
     return targets, info
