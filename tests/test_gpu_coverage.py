@@ -6,8 +6,10 @@
     the tolerances of a stand-alone run;
   * the external force types 1, 2 and 3 (src/kernel_ext_force.cu:22-72) through
     prepare and 5 steps, and through the stand-alone calculate_external_forces;
-  * a simulation with more colloids than the pair pass stages in LDS (40 > 32:
-    the colloid states are read from global memory), FP64 and FP32 forms;
+  * simulations with more colloids than the pair pass stages in LDS (40 and
+    the reference's maximum, 1000 > 32: the colloid states are read from
+    global memory; all-pairs colloid repulsion and magnetic forces), FP64 and
+    FP32 forms;
   * the cell-offset scan through k_scan_tile_sums + k_scan_add (forced at small
     grids by SPH_SCAN_SUM_TILES=1): the cell list bit-exact against the oracle
     and every field bitwise equal to the default path, single and batched;
@@ -15,6 +17,8 @@
     overflows (capacity 8): the forces stay those of the force pass and the
     gradient is the oracle's, with the overflow queue emptied before each pass.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -121,20 +125,34 @@ def test_external_force_types(case, tmp_path):
     assert np.abs(s.get("fx") - f0).max() > 0
 
 
-def _many_colloids_cfg():
-    R = 0.03
-    cfg = helpers.cfg_couette(n=96, dt=2e-5, Vwall_bottom=-0.2, Vwall_top=0.2)
-    cfg.update(N_colloids=40, coll_R=R, coll_rho=1.0, coll_move=0,
-               coll_x=inputs.place_colloids(40, R, [1.0, 1.0], 2, True, seed=9, wall_gap=3.0),
+# (count, R, lattice n, dt, magnetic cut-off, F0_magnetic): 40 colloids, more than the
+# pair pass stages in LDS; 1000, the reference's MAX_COLLOIDS (src/config.h:22),
+# ~3 lattice spacings in radius on a 300 x 300 lattice (88,593 particles)
+MANY_COLLOIDS = {40: (0.03, 96, 2e-5, 0.1, 1e-4), 1000: (0.0105, 300, 5e-6, 0.035, 1e-7)}
+
+
+@functools.lru_cache(maxsize=None)
+def _colloid_centres(count):
+    R = MANY_COLLOIDS[count][0]
+    return tuple(map(tuple, inputs.place_colloids(count, R, [1.0, 1.0], 2, True, seed=9,
+                                                  wall_gap=3.0)))
+
+
+def _many_colloids_cfg(count=40):
+    R, n, dt, rmag, f0 = MANY_COLLOIDS[count]
+    cfg = helpers.cfg_couette(n=n, dt=dt, Vwall_bottom=-0.2, Vwall_top=0.2)
+    cfg.update(N_colloids=count, coll_R=R, coll_rho=1.0, coll_move=0,
+               coll_x=[list(c) for c in _colloid_centres(count)],
                coll_repulsion_cuton=0.0, F0_repulsion=1.0, tau_repulsion=30.0,
-               cutoff_magnetic=0.1, F0_magnetic=1e-4, omega_magnetic=3.0)
+               cutoff_magnetic=rmag, F0_magnetic=f0, omega_magnetic=3.0)
     return cfg
 
 
+@pytest.mark.parametrize("count", [40, 1000])
 @pytest.mark.parametrize("precision", [sph_system.FP64, sph_system.FP32])
-def test_more_colloids_than_the_lds_stage(precision, tmp_path):
-    s = helpers.make_system(_many_colloids_cfg(), str(tmp_path), precision)
-    assert s.params.N_colloids == 40
+def test_more_colloids_than_the_lds_stage(precision, count, tmp_path):
+    s = helpers.make_system(_many_colloids_cfg(count), str(tmp_path), precision)
+    assert s.params.N_colloids == count
     helpers.perturb(s)
     o = oracle.Oracle.from_system(s)
     s.prepare()
@@ -146,7 +164,7 @@ def test_more_colloids_than_the_lds_stage(precision, tmp_path):
         s.step(5)
         o.step(5)
         s.synchronize()
-        _check_state(s, o, label="40 colloids")
+        _check_state(s, o, label=f"{count} colloids")
     else:
         assert relerr(s.get("dens"), o.get("dens")) <= 1e-6
         for n in comps("gradv", d) + comps("dcdt", d):
