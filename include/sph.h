@@ -193,6 +193,22 @@ int sph_print_output(sph_handle h, int step, const char *out_dir);
 /* Writes the restart file read by new_sim == 1 (column layout of
  * src/system_initialize_particles_old_sim.cpp:35-63). */
 int sph_write_restart(sph_handle h, const char *path);
+/* Exact binary checkpoint.  No reference counterpart: the reference's only
+ * restart is the input_particles.in above, whose values it parses as float
+ * and which leaves out forces, dc/dt and the colloids' velocities, spins and
+ * orientations (src/system_initialize.cpp:185-201).  Writes the parameters,
+ * the step counter and every particle, colloid and wall array at full
+ * precision, in ID order; SPH_ERR_IO when the file cannot be written. */
+int sph_checkpoint_write(sph_handle h, const char *path);
+/* Replaces the handle's whole state with a checkpoint written by a handle of
+ * the same precision form (SPH_ERR_ARG otherwise; SPH_ERR_IO for a missing,
+ * foreign or damaged file, leaving the handle as it was).  No read_input or
+ * initialize is needed; device state is dropped and rebuilt by the next
+ * device call.  sph_step then continues the run from the saved state - its
+ * first half kick uses the saved forces and dc/dt, as the uninterrupted run's
+ * would - with neighbour lists rebuilt at that step, so the two runs agree to
+ * rounding (neighbour sums in another order), not bit for bit. */
+int sph_checkpoint_read(sph_handle h, const char *path);
 
 /* ---- instrumentation ----------------------------------------------------- */
 /* When enabled, each device phase is bracketed by HIP events on the handle's

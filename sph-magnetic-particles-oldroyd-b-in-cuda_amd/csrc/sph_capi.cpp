@@ -365,6 +365,35 @@ int sph_write_restart(sph_handle h, const char *path) {
   return sph::write_restart(h->hs, path, h->err);
 }
 
+int sph_checkpoint_write(sph_handle h, const char *path) {
+  if (!h || !path) return SPH_ERR_ARG;
+  if (!h->hs.initialized) { h->err = "the system is not initialized"; return SPH_ERR_STATE; }
+  if (h->dev) {
+    int rc = sync_dev(h);
+    if (rc) return rc;
+    // every device-held field, in ID order (walls only exist with walls)
+    for (int k = 0; k < sph::F_NUM; ++k)
+      if ((rc = h->dev->download_field(0, h->hs, sph::kPFieldNames[k], h->err))) return rc;
+    for (int k = 0; k < sph::C_NUM && h->hs.p.N_colloids > 0; ++k)
+      if ((rc = h->dev->download_field(0, h->hs, sph::kCFieldNames[k], h->err))) return rc;
+    if ((rc = pull(h, {"type"}))) return rc;
+    if (h->hs.p.wall && (rc = pull(h, {"fx_wall", "fy_wall", "fz_wall"}))) return rc;
+  }
+  return sph::write_checkpoint(h->hs, h->precision, h->step, path, h->err);
+}
+
+int sph_checkpoint_read(sph_handle h, const char *path) {
+  if (!h || !path) return SPH_ERR_ARG;
+  int64_t step = 0;
+  int rc = sph::read_checkpoint(h->hs, h->precision, &step, path, h->err);
+  if (rc) return rc;
+  // the next device call uploads the restored state into a new device state
+  // (its lists are rebuilt at the first step)
+  h->dev.reset();
+  h->step = (int)step;
+  return SPH_OK;
+}
+
 int sph_set_timing(sph_handle h, int enable) {
   if (!h) return SPH_ERR_ARG;
   h->timing = enable != 0;

@@ -3235,12 +3235,17 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemcpy(id_[0].p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(rank_.p, iota.data(), N_ * sizeof(int), hipMemcpyHostToDevice));
     id_host_ = iota;
-    // derived fields start at zero (reference: cudaMemset, system_copy_pointers_to_device.cpp:190-256)
+    // derived fields as the host state holds them: zero after initialize
+    // (reference: cudaMemset, system_copy_pointers_to_device.cpp:190-256), the
+    // saved values after a checkpoint is read (the next VV1 kicks with the
+    // saved forces and dc/dt)
     static const int ffield[3] = {F_FX, F_FY, F_FZ};
     for (int a = 0; a < 3; ++a)
       if ((rc = put_all(f_[a].p, ffield[a]))) return rc;
-    for (int a = 0; a < 6; ++a) HIPCHK(hipMemset(dcdt_[a].p, 0, N_ * sizeof(R)));
-    for (int a = 0; a < 9; ++a) HIPCHK(hipMemset(gv_[a].p, 0, N_ * sizeof(R)));
+    for (int a = 0; a < 6; ++a)
+      if ((rc = put_all(dcdt_[a].p, F_DCDT_XX + a))) return rc;
+    for (int a = 0; a < 9; ++a)
+      if ((rc = put_all(gv_[a].p, F_GXX + a))) return rc;
     // The second buffer set starts as a copy, so rows a failed reorder leaves
     // unwritten still hold finite values.
     HIPCHK(hipMemcpy(x_[1].p, x_[0].p, N_ * sizeof(double), hipMemcpyDeviceToDevice));
@@ -3253,8 +3258,7 @@ class DeviceImpl final : public Device {
     HIPCHK(hipMemcpy(mass_[1].p, mass_[0].p, N_ * sizeof(R), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(type_[1].p, type_[0].p, N_ * sizeof(int), hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(id_[1].p, id_[0].p, N_ * sizeof(int), hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemset(dens_.p, 0, N_ * sizeof(R)));
-    HIPCHK(hipMemset(press_.p, 0, N_ * sizeof(R)));
+    if ((rc = put_all(dens_.p, F_DENS)) || (rc = put_all(press_.p, F_PRESS))) return rc;
     HIPCHK(hipMemset(count_.p, 0, ncells_ * sizeof(int)));
     HIPCHK(hipMemset(cs_.p, 0, (ncells_ + 1) * sizeof(int)));
     HIPCHK(hipMemset(flag_.p, 0, S * sizeof(int)));

@@ -780,4 +780,160 @@ int write_restart(const HostState &s, const std::string &path, std::string &err)
   return SPH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Binary checkpoint: the whole host state at full precision.  Layout (native
+// little-endian): the 8-byte magic, int32 precision, int32 sizeof(sph_params_t),
+// the parameter block, int64 step, then arrays as records (char name[16],
+// int32 kind 0 = double / 1 = int32, int64 count, the values), closed by a
+// record named "end" with no values.
+namespace {
+
+constexpr char kCkptMagic[8] = {'S', 'P', 'H', 'C', 'K', 'P', 'T', '\x01'};
+
+struct CkptArray {
+  std::string name;
+  std::vector<double> *d = nullptr;
+  std::vector<int32_t> *i = nullptr;
+  int64_t count;  // entries written and expected: fixed by the parameters
+  int64_t alloc;  // storage the state keeps (Nmax for the per-particle arrays)
+};
+
+// Every array of the state with its length under the parameters p (the
+// lengths initialize gives them).
+std::vector<CkptArray> ckpt_arrays(HostState &s, const sph_params_t &p) {
+  std::vector<CkptArray> a;
+  const int64_t nc = p.N_colloids, nl = p.Nlist_colloids;
+  for (int k = 0; k < F_NUM; ++k) a.push_back({kPFieldNames[k], &s.pf[k], nullptr, p.N, p.Nmax});
+  a.push_back({"type", nullptr, &s.type, p.N, p.Nmax});
+  for (int k = 0; k < C_NUM; ++k) a.push_back({kCFieldNames[k], &s.cf[k], nullptr, nc, nc});
+  a.push_back({"coll_x_data", &s.coll_x_data, nullptr, 3 * nc, 3 * nc});
+  a.push_back({"walls_list", nullptr, &s.walls_list, p.Nlist_walls, p.Nlist_walls});
+  a.push_back({"walls_start", nullptr, &s.walls_start, p.Nwalls, p.Nwalls});
+  a.push_back({"colloids_list", nullptr, &s.colloids_list, nl, nl});
+  a.push_back({"colloids_start", nullptr, &s.colloids_start, nc, nc});
+  a.push_back({"x_center", &s.x_center, nullptr, nl, nl});
+  a.push_back({"y_center", &s.y_center, nullptr, nl, nl});
+  a.push_back({"z_center", &s.z_center, nullptr, nl, nl});
+  return a;
+}
+
+bool put(FILE *f, const void *p, size_t n) { return n == 0 || std::fwrite(p, 1, n, f) == n; }
+bool get(FILE *f, void *p, size_t n) { return n == 0 || std::fread(p, 1, n, f) == n; }
+
+bool put_record(FILE *f, const std::string &name, int32_t kind, int64_t count, const void *v) {
+  char nm[16] = {0};
+  std::strncpy(nm, name.c_str(), sizeof(nm) - 1);
+  const size_t bytes = (size_t)count * (kind == 0 ? sizeof(double) : sizeof(int32_t));
+  return put(f, nm, sizeof(nm)) && put(f, &kind, sizeof(kind)) && put(f, &count, sizeof(count)) &&
+         put(f, v, bytes);
+}
+
+}  // namespace
+
+int write_checkpoint(const HostState &cs, int precision, int64_t step, const std::string &path,
+                     std::string &err) {
+  HostState &s = const_cast<HostState &>(cs);  // (arrays are only read)
+  if (!s.initialized) { err = "checkpoint: the system is not initialized"; return SPH_ERR_STATE; }
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) { err = "checkpoint: cannot open " + path; return SPH_ERR_IO; }
+  const int32_t prec = precision, psize = (int32_t)sizeof(sph_params_t);
+  bool ok = put(f, kCkptMagic, sizeof(kCkptMagic)) && put(f, &prec, sizeof(prec)) &&
+            put(f, &psize, sizeof(psize)) && put(f, &s.p, sizeof(s.p)) &&
+            put(f, &step, sizeof(step));
+  double fw[6];
+  for (int a = 0; a < 3; ++a)
+    for (int w = 0; w < 2; ++w) fw[2 * a + w] = s.fwall[a][w];
+  ok = ok && put_record(f, "fwall", 0, 6, fw);
+  for (const CkptArray &a : ckpt_arrays(s, s.p)) {
+    const size_t have = a.d ? a.d->size() : a.i->size();
+    if ((int64_t)have < a.count) {
+      std::fclose(f);
+      err = "checkpoint: array " + a.name + " is shorter than the state";
+      return SPH_ERR_STATE;
+    }
+    ok = ok && put_record(f, a.name, a.d ? 0 : 1, a.count,
+                          a.d ? (const void *)a.d->data() : (const void *)a.i->data());
+  }
+  ok = ok && put_record(f, "end", 0, 0, nullptr);
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) { err = "checkpoint: write failed on " + path; return SPH_ERR_IO; }
+  return SPH_OK;
+}
+
+int read_checkpoint(HostState &out, int precision, int64_t *step, const std::string &path,
+                    std::string &err) {
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) { err = "checkpoint: cannot open " + path; return SPH_ERR_IO; }
+  auto fail = [&](const std::string &m) {
+    std::fclose(f);
+    err = "checkpoint " + path + ": " + m;
+    return SPH_ERR_IO;
+  };
+  char magic[8];
+  int32_t prec = -1, psize = 0;
+  HostState s;
+  int64_t st = 0;
+  if (!get(f, magic, sizeof(magic)) || std::memcmp(magic, kCkptMagic, sizeof(magic)))
+    return fail("not a checkpoint of this library");
+  if (!get(f, &prec, sizeof(prec)) || !get(f, &psize, sizeof(psize)) ||
+      psize != (int32_t)sizeof(sph_params_t) || !get(f, &s.p, sizeof(s.p)) ||
+      !get(f, &st, sizeof(st)))
+    return fail("truncated header");
+  if (prec != precision) {
+    std::fclose(f);
+    err = "checkpoint " + path + " holds the other precision form";
+    return SPH_ERR_ARG;
+  }
+  const sph_params_t &p = s.p;
+  if (p.N < 0 || p.N > p.Nmax || p.N_colloids < 0 || p.Nlist_colloids < 0 ||
+      p.Nlist_walls < 0 || p.Nwalls < 0 || p.Nwalls > 2 || (p.dim != 2 && p.dim != 3))
+    return fail("inconsistent parameters");
+  std::vector<CkptArray> arrays = ckpt_arrays(s, p);
+  for (CkptArray &a : arrays) {
+    if (a.d) a.d->assign(a.alloc, 0.0);
+    else a.i->assign(a.alloc, 0);
+  }
+  std::vector<bool> seen(arrays.size(), false);
+  bool fwall = false;
+  for (;;) {
+    char nm[16];
+    int32_t kind;
+    int64_t count;
+    if (!get(f, nm, sizeof(nm)) || !get(f, &kind, sizeof(kind)) || !get(f, &count, sizeof(count)))
+      return fail("truncated record");
+    nm[15] = 0;
+    const std::string name = nm;
+    if (name == "end") break;
+    if ((kind != 0 && kind != 1) || count < 0 || count > (int64_t)1 << 32)
+      return fail("bad record " + name);
+    if (name == "fwall") {
+      double fw[6];
+      if (kind != 0 || count != 6 || !get(f, fw, sizeof(fw))) return fail("bad record fwall");
+      for (int a = 0; a < 3; ++a)
+        for (int w = 0; w < 2; ++w) s.fwall[a][w] = fw[2 * a + w];
+      fwall = true;
+      continue;
+    }
+    size_t k = 0;
+    while (k < arrays.size() && arrays[k].name != name) ++k;
+    if (k == arrays.size()) return fail("unknown record " + name);
+    CkptArray &a = arrays[k];
+    if (seen[k] || (a.d != nullptr) != (kind == 0) || count != a.count)
+      return fail("record " + name + " does not fit the parameters");
+    const bool got = a.d ? get(f, a.d->data(), (size_t)count * sizeof(double))
+                         : get(f, a.i->data(), (size_t)count * sizeof(int32_t));
+    if (!got) return fail("truncated " + name);
+    seen[k] = true;
+  }
+  std::fclose(f);
+  for (size_t k = 0; k < arrays.size(); ++k)
+    if (!seen[k]) { err = "checkpoint " + path + ": no record " + arrays[k].name; return SPH_ERR_IO; }
+  if (!fwall) { err = "checkpoint " + path + ": no record fwall"; return SPH_ERR_IO; }
+  s.input_read = s.initialized = true;
+  s.input_dir = out.input_dir;
+  out = std::move(s);
+  *step = st;
+  return SPH_OK;
+}
+
 }  // namespace sph

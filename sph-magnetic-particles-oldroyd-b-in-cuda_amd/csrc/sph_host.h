@@ -66,5 +66,14 @@ int write_colloids(const HostState &s, int step, const std::string &dir, std::st
 int append_macro(const HostState &s, int step, double ke, const std::string &dir,
                  std::string &err);
 int write_restart(const HostState &s, const std::string &path, std::string &err);
+// Binary checkpoint of the whole host state (parameters, every particle,
+// colloid and wall array and list, at full precision) and the step counter.
+// read_checkpoint replaces `s` only when the file reads completely; it keeps
+// s.input_dir.  SPH_ERR_IO on a missing, foreign or damaged file, SPH_ERR_ARG
+// when the file holds the other precision form.
+int write_checkpoint(const HostState &s, int precision, int64_t step, const std::string &path,
+                     std::string &err);
+int read_checkpoint(HostState &s, int precision, int64_t *step, const std::string &path,
+                    std::string &err);
 
 }  // namespace sph

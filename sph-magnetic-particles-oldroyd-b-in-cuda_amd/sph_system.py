@@ -26,6 +26,8 @@ STATUS = {
     4: "SPH_ERR_GEOMETRY", 5: "SPH_ERR_DEVICE", 6: "SPH_ERR_OUT_OF_GRID",
     7: "SPH_ERR_NAN", 8: "SPH_ERR_STATE", 9: "SPH_ERR_CAPACITY",
 }
+(OK, ERR_ARG, ERR_IO, ERR_PARSE, ERR_GEOMETRY, ERR_DEVICE, ERR_OUT_OF_GRID, ERR_NAN, ERR_STATE,
+ ERR_CAPACITY) = range(10)
 
 # The C-ABI entry points (include/sph.h).  Tests check that each is exported.
 ENTRY_POINTS = [
@@ -37,7 +39,8 @@ ENTRY_POINTS = [
     "sph_calculate_force_colloids", "sph_calculate_forces", "sph_V_Verlet_step1",
     "sph_V_Verlet_step2", "sph_set_step", "sph_prepare", "sph_step", "sph_synchronize",
     "sph_current_step", "sph_run", "sph_kinetic_energy", "sph_print_output",
-    "sph_write_restart", "sph_set_timing", "sph_get_timing", "sph_get_counters",
+    "sph_write_restart", "sph_checkpoint_write", "sph_checkpoint_read", "sph_set_timing",
+    "sph_get_timing", "sph_get_counters",
     "sph_observables",
     "sph_batch_create", "sph_batch_destroy", "sph_batch_last_error", "sph_batch_add",
     "sph_batch_size", "sph_batch_prepare", "sph_batch_step", "sph_batch_synchronize",
@@ -164,6 +167,10 @@ def library(path=None):
     lib.sph_run.argtypes = [vp, cp]
     lib.sph_write_restart.restype = i32
     lib.sph_write_restart.argtypes = [vp, cp]
+    lib.sph_checkpoint_write.restype = i32
+    lib.sph_checkpoint_write.argtypes = [vp, cp]
+    lib.sph_checkpoint_read.restype = i32
+    lib.sph_checkpoint_read.argtypes = [vp, cp]
     lib.sph_observables.restype = i32
     lib.sph_observables.argtypes = [vp, vp]
     lib.sph_get_timing.restype = i32
@@ -375,6 +382,17 @@ class System:
 
     def write_restart(self, path):
         self._check(self.lib.sph_write_restart(self.h, os.fsencode(path)), "write_restart")
+
+    def write_checkpoint(self, path):
+        """Exact binary checkpoint of the whole state and the step counter."""
+        self._check(self.lib.sph_checkpoint_write(self.h, os.fsencode(path)),
+                    "checkpoint_write")
+
+    def read_checkpoint(self, path):
+        """Replaces the whole state with a checkpoint (no read_input or
+        initialize needed); the next step continues the saved run."""
+        self._check(self.lib.sph_checkpoint_read(self.h, os.fsencode(path)), "checkpoint_read")
+        self._params = None
 
     def set_timing(self, on=True):
         self._call("sph_set_timing", 1 if on else 0)
