@@ -553,21 +553,26 @@ __device__ __forceinline__ void for_each_reach(const sph_params_t &P, const int 
   }
 }
 
-// Windows of the staged FP32 density walk.  At a list build each window tile
+// Windows of the staged density walk.  At a list build each window tile
 // (kWinTile consecutive particles of a simulation, from its first; a workgroup
 // of the passes lies inside one) gets a window: the sorted-index ranges of the
 // cells within one cell per axis of the cells its particles occupy (periodic
 // images folded in, rows beyond a wall dropped), merged.  Every list entry also
 // gets its slot in the window, or kNoSlot for the few entries that a build's
 // reach adds from the second ring of cells.  The density pass copies the
-// window's fixed-point positions into LDS once per workgroup and reads each
-// entry's position from there: the gathers no longer go through the vector L1
+// window's positions (FP32 form: fixed point; FP64 form: doubles) into LDS
+// once per workgroup and reads each entry's position from there: the gathers
+// no longer go through the vector L1
 // one cache line per lane, and a position is fetched once per tile instead of
 // once per particle that lists it.  Windows do not depend on the launch shape.
 constexpr int kWinTile = 256;                 // particles per window tile
 constexpr int kWinRanges = 24;                // merged index ranges per window
 constexpr int kWinInts = 2 + 2 * kWinRanges;  // [ranges | -1, positions, (first, count)...]
 constexpr int kStageCap = 3584;               // positions per window (3 x 4 B each in LDS)
+// FP64 form: 3 x 8 B each in LDS; two workgroups of the density pass per CU
+constexpr int kStageCap64 = 3376;
+template <class R>
+constexpr int kStageCapOf = sizeof(R) == 4 ? kStageCap : kStageCap64;
 constexpr unsigned kNoSlot = 0xFFFFu;
 
 // The window of a tile whose particles occupy cells c0 .. c1 (the simulation's
@@ -1522,7 +1527,108 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       if (vok) d = cw * (R(66) + tsum);  // (W(0) = 66 cw: the particle itself)
     }
   } else {
-    if (VL.idx) {
+    // (uniform: the tile's window, or -1 for the unstaged walk)
+    const int *__restrict__ gw =
+        VL.idx && VL.slot ? VL.win + (size_t)window_of(T, VL.wbase, b, sim) * kWinInts : nullptr;
+    const int nwin = gw ? gw[0] : -1;
+    // the LDS of the staged walk's window or of the unstaged walk's position
+    // hand-over (a workgroup takes one of the two walks)
+    __shared__ __attribute__((aligned(16))) double lds64[3 * kStageCap64];
+    if (nwin >= 0) {
+      // FP64 form, staged walk (as the FP32 one above): the window's positions
+      // (the doubles of the packed position copy) are copied into LDS, then
+      // each entry's position is read from its slot, entries of the second
+      // ring of cells (kNoSlot) from memory.  The same positions, tests and
+      // sums in the same order as the unstaged walk below: the same densities.
+      double *wxd = lds64, *wyd = lds64 + kStageCap64, *wzd = lds64 + 2 * kStageCap64;
+      __shared__ int wr[kWinInts];
+      if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
+      __syncthreads();
+      const int total = wr[1];
+      int r = 0, ro = 0;
+#pragma unroll 1
+      for (int p0 = (int)threadIdx.x; p0 < total; p0 += 4 * (int)blockDim.x) {
+        double4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = p0 + u * (int)blockDim.x;
+          if (p < total) {
+            while (p >= ro + wr[3 + 2 * r]) ro += wr[3 + 2 * r++];
+            q[u] = S.pd[wr[2 + 2 * r] + (p - ro)];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = p0 + u * (int)blockDim.x;
+          if (p < total) {
+            wxd[p] = q[u].x;
+            wyd[p] = q[u].y;
+            if (DIM == 3) wzd[p] = q[u].z;
+          }
+        }
+      }
+      __syncthreads();
+      const int vc = vok ? VL.count[i] : 0;
+      const int4 *__restrict__ vl = VL.idx + i;
+      const int pb = T.pbeg[sim];
+      const uint4 *__restrict__ sl = VL.slot + i;
+      const int capq = VL.cap >> 3;
+      auto slot_group = [&](int gq) { return ld_stream(&sl[(size_t)min(gq, capq - 1) * N]); };
+      uint4 sg = slot_group(0);
+      if (vok) d = sph_W<R>(R(0), rinv, cw);  // the particle itself (src/kernel_densities.cu:95-99)
+      unsigned mw = 0u;
+#pragma unroll 1
+      for (int g = 0; g < vc; g += 8) {
+        const uint4 sn = slot_group((g >> 3) + 1);
+        const unsigned w4[4] = {sg.x, sg.y, sg.z, sg.w};
+        double px[8], py[8], pz[8];
+        bool far = false;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+          const unsigned sc = min(sk, (unsigned)(kStageCap64 - 1));
+          px[u] = wxd[sc];
+          py[u] = wyd[sc];
+          pz[u] = DIM == 3 ? wzd[sc] : 0.0;
+          far |= sk == kNoSlot && g + u < vc;
+        }
+        if (far) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+            if (sk == kNoSlot && g + u < vc) {
+              const int4 q4 = vl[(size_t)((g + u) >> 2) * N];
+              const int cq = (g + u) & 3;
+              const int j = cq == 0 ? q4.x : cq == 1 ? q4.y : cq == 2 ? q4.z : q4.w;
+              const double4 pj = S.pd[pb + j];
+              px[u] = pj.x;
+              py[u] = pj.y;
+              pz[u] = pj.z;
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (g + u < vc) {
+            R rs[3];
+            pair_sep<R, DIM, WALL>(bx, xi, yi, zi, px[u], py[u], DIM == 3 ? pz[u] : 0.0, rs);
+            const R r2 = DIM == 3 ? rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2]
+                                  : rs[0] * rs[0] + rs[1] * rs[1];
+            if (r2 < rcutsq) {
+              d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
+              mw |= 1u << ((g + u) & 31);
+              cnt++;
+            }
+          }
+        }
+        if (((g + 8) & 31) == 0 || g + 8 >= vc) {
+          VL.mask[(size_t)(g >> 5) * N + i] = mw;
+          mw = 0u;
+        }
+        sg = sn;
+      }
+      if (!live) return;
+    } else if (VL.idx) {
       // FP64 form, Verlet list: the hits in list order, W summed as they come; all
       // lanes of the wave step through the list groups (four entries) together (a lane
       // past its list or not on the Verlet path has no entries), so index loads
@@ -1536,8 +1642,8 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       // memory access).
       typedef unsigned u4 __attribute__((ext_vector_type(4)));
       constexpr int kSlot = 3;  // 48-byte slots: conflict-free LDS rows
-      __shared__ u4 stage[kBlock / 64][64][kSlot];
-      u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
+      static_assert(sizeof(u4) * kBlock * kSlot <= sizeof(lds64), "the hand-over fits");
+      u4(&st)[64][kSlot] = reinterpret_cast<u4(*)[64][kSlot]>(lds64)[threadIdx.x >> 6];
       const int lane = threadIdx.x & 63, hb = lane & 1, pa = lane & ~1;
       const int vc = vok ? VL.count[i] : 0;
       int vmax = vc;
@@ -1744,14 +1850,13 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const int *__restrict__ cs = csg + T.cell_off[sim];
   const int N = T.ntotal;
   const int iend = T.pbeg[sim + 1];
-  // FP32 form with the staged density walk: the window of this workgroup's
+  // With the staged density walk: the window of this workgroup's
   // window tile, from the cells of the tile's first and last particle (the
   // reorder of this step sorted them by these cells), before any lane leaves;
   // the tile's first workgroup stores it
-  constexpr bool kSlots = sizeof(R) == 4;
-  __shared__ int wwin[kSlots ? kWinInts : 1];
-  __shared__ int woff[kSlots ? kWinRanges : 1];
-  const bool slots = kSlots && VL.slot;
+  __shared__ int wwin[kWinInts];
+  __shared__ int woff[kWinRanges];
+  const bool slots = VL.slot != nullptr;
   if (slots) {
     if (threadIdx.x < 64) {
       const int f = T.pbeg[sim] + (T.blk_first[b] - T.pbeg[sim]) / kWinTile * kWinTile;
@@ -3136,7 +3241,7 @@ class DeviceImpl final : public Device {
     vcap_ = ((int)(1.5 * vmean) + 64 + 7) / 8 * 8;
     if (const char *e = std::getenv("SPH_VERLET_CAPACITY"))
       vcap_ = std::max(8, (std::atoi(e) + 7) / 8 * 8);
-    // Staged FP32 density walk, 3D (SPH_DENSITY_STAGE=0 turns it off, =2 turns
+    // Staged density walk, 3D (SPH_DENSITY_STAGE=0 turns it off, =2 turns
     // it on in 2D too; SPH_STAGE_CAP lowers the positions a window may hold,
     // so that more tiles take the unstaged walk; results are the same either
     // way).  In 2D (about 30 entries per particle) the window copy and the
@@ -3144,10 +3249,10 @@ class DeviceImpl final : public Device {
     // they replace: a 55-simulation sweep batch stepped 10 % slower.
     int stage_mode = 1;
     if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_mode = std::atoi(e);
-    stage_ = sizeof(R) == 4 && verlet_ok_ && stage_mode != 0 && (P_.dim == 3 || stage_mode == 2);
-    stage_cap_ = kStageCap;
+    stage_ = verlet_ok_ && stage_mode != 0 && (P_.dim == 3 || stage_mode == 2);
+    stage_cap_ = kStageCapOf<R>;
     if (const char *e = std::getenv("SPH_STAGE_CAP"))
-      stage_cap_ = std::min(kStageCap, std::max(0, std::atoi(e)));
+      stage_cap_ = std::min(kStageCapOf<R>, std::max(0, std::atoi(e)));
     if (verlet_ok_) {
       HIPCHK(vl_idx_.alloc((size_t)(vcap_ / 4) * N_));
       if (stage_) {
@@ -4141,7 +4246,7 @@ class DeviceImpl final : public Device {
   DevBuf<int> vl_win_;     // VList::win
   DevBuf<int> wbase_;      // VList::wbase
   bool stage_ = false;
-  int stage_cap_ = kStageCap;
+  int stage_cap_ = kStageCapOf<R>;
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
   DevBuf<int> any_;  // [1] batch rebuild flag (VerletCtl::any)

@@ -383,9 +383,10 @@ def test_observables(case, tmp_path):
     np.testing.assert_allclose(a[~both_nan], b[~both_nan], rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("precision", [sph_system.FP32, sph_system.FP64], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("case", ["colloids3d", "sine3d", "couette2d"])
-def test_staged_density_walk_is_the_unstaged_one(case, tmp_path, monkeypatch):
-    """The FP32 density walk reads neighbour positions from a per-tile LDS
+def test_staged_density_walk_is_the_unstaged_one(case, precision, tmp_path, monkeypatch):
+    """The density walk reads neighbour positions from a per-tile LDS
     window (SPH_DENSITY_STAGE=2: in 2D too; =0 reads them from memory;
     SPH_STAGE_CAP=400 leaves the tiles with larger windows to the unstaged
     walk).  The same
@@ -403,7 +404,7 @@ def test_staged_density_walk_is_the_unstaged_one(case, tmp_path, monkeypatch):
             monkeypatch.delenv(k, raising=False)
         for k, v in mode.items():
             monkeypatch.setenv(k, v)
-        s = helpers.make_system(cfg, str(tmp_path), sph_system.FP32, noise=1e-2,
+        s = helpers.make_system(cfg, str(tmp_path), precision, noise=1e-2,
                                 name=f"m{len(outs)}")
         helpers.perturb(s)
         s.prepare()
