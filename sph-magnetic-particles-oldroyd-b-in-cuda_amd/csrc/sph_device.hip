@@ -2576,12 +2576,12 @@ __global__ __launch_bounds__(kBlock) void k_ext(SimTable T, Particles<R> S, Deri
 // are deterministic and the same in a batch as alone.
 constexpr int kWallBlock = 256, kWallChunk = 1024;
 
+// One workgroup of the wall sums: chunk `ch` of wall w of simulation `sim`
+// (the list holds global IDs).
 template <class R>
-__global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, Bodies B,
-                                                      const int *__restrict__ rank, double *part,
-                                                      unsigned *cnt, int maxch) {
-  // block (chunk, w, s): chunk of wall w of simulation s; the list holds global IDs
-  const int ch = blockIdx.x, w = blockIdx.y, sim = blockIdx.z;
+__device__ __forceinline__ void walls_block(const SimTable &T, const Derived<R> &D, const Bodies &B,
+                                            const int *__restrict__ rank, double *part,
+                                            unsigned *cnt, int maxch, int ch, int w, int sim) {
   const int j0 = T.wrange[2 * sim + w], j1 = T.wrange[2 * sim + w + 1];
   const int nch = max(1, (j1 - j0 + kWallChunk - 1) / kWallChunk);
   if (ch >= nch) return;
@@ -2631,12 +2631,22 @@ __global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, 
 }
 
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S, Derived<R> D,
-                                                     Bodies Ball, const int *__restrict__ rank,
-                                                     const double *__restrict__ mag_all,
-                                                     int kick) {
-  // one block per colloid of the batch; ci is its index in its simulation
-  const int g = blockIdx.x, sim = T.coll_sim[g];
+__global__ __launch_bounds__(kWallBlock) void k_walls(SimTable T, Derived<R> D, Bodies B,
+                                                      const int *__restrict__ rank, double *part,
+                                                      unsigned *cnt, int maxch) {
+  // block (chunk, w, s)
+  walls_block<R>(T, D, B, rank, part, cnt, maxch, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// One workgroup of the colloid pass: colloid g of the batch.
+template <class R>
+__device__ __forceinline__ void colloid_block(const SimTable &T, const Particles<R> &S,
+                                              const Derived<R> &D, const Bodies &Ball,
+                                              const int *__restrict__ rank,
+                                              const double *__restrict__ mag_all, int kick,
+                                              int g) {
+  // ci is the colloid's index in its simulation
+  const int sim = T.coll_sim[g];
   const sph_params_t &P = T.P[sim];
   const Bodies B = bodies_of(Ball, T, sim);
   const int ci = g - T.coll_off[sim], ncoll = P.N_colloids;
@@ -2715,6 +2725,36 @@ __global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S,
       B.s[C_OX][ci] += hdtI * acc[3];
       B.s[C_OY][ci] += hdtI * acc[4];
     }
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_colloids(SimTable T, Particles<R> S, Derived<R> D,
+                                                     Bodies Ball, const int *__restrict__ rank,
+                                                     const double *__restrict__ mag_all,
+                                                     int kick) {
+  colloid_block<R>(T, S, D, Ball, rank, mag_all, kick, blockIdx.x);  // one block per colloid
+}
+
+// The wall sums and the colloid pass of a step in one launch (they read the
+// forces of disjoint particle sets and write disjoint outputs; both are
+// latency-bound, so side by side they take about the time of one): workgroups
+// [0, ncoll) are the colloids', the others the wall chunks (chunk fastest,
+// then wall, then simulation).
+static_assert(kWallBlock == kBlock, "one workgroup size for both");
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_bodies(SimTable T, Particles<R> S, Derived<R> D,
+                                                   Bodies Ball, const int *__restrict__ rank,
+                                                   const double *__restrict__ mag_all, int kick,
+                                                   int ncoll, double *part, unsigned *cnt,
+                                                   int maxch) {
+  const int b = blockIdx.x;
+  if (b < ncoll) {
+    colloid_block<R>(T, S, D, Ball, rank, mag_all, kick, b);
+  } else {
+    const int k = b - ncoll;
+    walls_block<R>(T, D, Ball, rank, part, cnt, maxch, k % maxch, (k / maxch) & 1,
+                   k / (2 * maxch));
   }
 }
 
@@ -3696,6 +3736,14 @@ class DeviceImpl final : public Device {
     int rc;
     if ((rc = density(err))) return rc;
     if ((rc = pairs_all(false, err, true))) return rc;
+    if (any_walls_ && ncoll_ > 0) {
+      Mark m(this, PH_BODIES);
+      k_bodies<R><<<ncoll_ + wmaxch_ * 2 * nsim(), kBlock, 0, stream_>>>(
+          table(), parts(cur_), derived(), bodies(), rank_.p, mag_.p, 1, ncoll_, wpart_.p,
+          wcnt_.p, wmaxch_);
+      HIPCHK(hipGetLastError());
+      return SPH_OK;
+    }
     if ((rc = force_walls(err))) return rc;
     return colloid_forces(true, err);
   }
