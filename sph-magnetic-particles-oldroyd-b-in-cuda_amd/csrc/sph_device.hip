@@ -1056,8 +1056,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SimTable T, const int *__res
 // smaller ID, i.e. the stable (cell, ID) order.  Gathers the moving state.
 // A simulation that is not rebuilt (or is held) keeps its order: with copy_others its state
 // is copied position for position into the other buffer set (a full reorder
-// flips the sets); without, it is left alone (a Verlet step copies the rebuilt
-// simulations back instead, k_copy_back).
+// flips the sets); without, it is left alone (a Verlet step reorders in place:
+// its list build copies the rebuilt simulations back, x v c m type ID; the
+// packed position copies, shared by both sets, are written here in the new
+// order).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
     SimTable T, Particles<R> src, Particles<R> dst, const int *__restrict__ cell,
@@ -1104,30 +1106,6 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
   rank[myid] = d;
   sorted_cell[d] = c;
   if (newpos) newpos[k] = d;
-}
-
-// A Verlet step reorders in place: the rebuilt simulations' state, gathered into
-// the other buffer set, is copied back (x v c m type ID; the packed position
-// copies were written in the new order by the gather).  Simulations that keep
-// their order are not touched.  Block 0 clears the batch rebuild flag: every
-// reader of it in this step has run.
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_copy_back(SimTable T, Particles<R> src,
-                                                      Particles<R> dst, int nc,
-                                                      const int *rebuild, int *any) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *any = 0;
-  const int b = blockIdx.x, sim = T.blk_sim[b];
-  const int p = T.blk_first[b] + threadIdx.x;
-  if (p >= T.pbeg[sim + 1] || !rebuild[sim] || T.halt[sim]) return;
-  dst.x[p] = src.x[p];
-  dst.y[p] = src.y[p];
-  dst.z[p] = src.z[p];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) dst.v[a][p] = src.v[a][p];
-  for (int a = 0; a < nc; ++a) dst.c[a][p] = src.c[a][p];
-  dst.mass[p] = src.mass[p];
-  dst.type[p] = src.type[p];
-  dst.id[p] = src.id[p];
 }
 
 // tmp[newpos[k]] = a[k]: a per-particle field carried into the new order.
@@ -1835,6 +1813,7 @@ __device__ __forceinline__ void set_step_of(const SimTable &T, int *steps, doubl
 // work for this step (one launch fewer per step).
 template <int DIM, bool WALL, class R>
 __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R> S,
+                                                         Particles<R> G, int nc, int *any,
                                                          const int *__restrict__ csg, VList VL,
                                                          VerletCtl V, int *steps, double *mag,
                                                          int step) {
@@ -1842,6 +1821,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < T.nsim) set_step_of(T, steps, mag, step, s);
   }
+  // (every reader of the batch rebuild flag in this step has run)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *any = 0;
   const int b = tile_of(T), sim = T.blk_sim[b];
   if (!V.rebuild[sim] || T.halt[sim]) return;
   const int i = T.blk_first[b] + threadIdx.x;
@@ -1850,6 +1831,21 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   const int *__restrict__ cs = csg + T.cell_off[sim];
   const int N = T.ntotal;
   const int iend = T.pbeg[sim + 1];
+  // The reorder of this step gathered the simulation into the other buffer set
+  // G in the new order (the packed position copies are shared by both sets):
+  // each thread copies its particle back into S, and every position the build
+  // reads comes from G, which no thread of this launch writes.
+  if (i < iend) {
+    S.x[i] = G.x[i];
+    S.y[i] = G.y[i];
+    S.z[i] = G.z[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S.v[a][i] = G.v[a][i];
+    for (int a = 0; a < nc; ++a) S.c[a][i] = G.c[a][i];
+    S.mass[i] = G.mass[i];
+    S.type[i] = G.type[i];
+    S.id[i] = G.id[i];
+  }
   // With the staged density walk: the window of this workgroup's
   // window tile, from the cells of the tile's first and last particle (the
   // reorder of this step sorted them by these cells), before any lane leaves;
@@ -1862,8 +1858,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
       const int f = T.pbeg[sim] + (T.blk_first[b] - T.pbeg[sim]) / kWinTile * kWinTile;
       const int l = min(f + kWinTile, iend) - 1;
       bool ok;
-      const int c0 = cell_of(P, S.x[f], S.y[f], DIM == 3 ? S.z[f] : 0.0, ok);
-      const int c1 = cell_of(P, S.x[l], S.y[l], DIM == 3 ? S.z[l] : 0.0, ok);
+      const int c0 = cell_of(P, G.x[f], G.y[f], DIM == 3 ? G.z[f] : 0.0, ok);
+      const int c1 = cell_of(P, G.x[l], G.y[l], DIM == 3 ? G.z[l] : 0.0, ok);
       window_build<DIM, WALL>(P, cs, c0, c1, VL.stage_cap, wwin, woff);
     }
     __syncthreads();
@@ -1889,7 +1885,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     wsb = woff[lo] - f;
     return (unsigned)(k + wsb);
   };
-  const double xi = S.x[i], yi = S.y[i], zi = DIM == 3 ? S.z[i] : 0.0;
+  const double xi = G.x[i], yi = G.y[i], zi = DIM == 3 ? G.z[i] : 0.0;
   int cx, cy, cz;
   cell_coords<DIM, WALL>(P, xi, yi, zi, cx, cy, cz);
   const double rv = P.rcut * (1.0 + (double)VL.skin_frac);
@@ -1907,7 +1903,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   // eight candidates' loads in flight at a time.  Entries are indices within
   // the simulation (k - pbeg), the offsets the passes' buffer loads take.
   CandSelf<R, DIM> me;
-  me.init(S, i, P);
+  me.init(G, i, P);
   auto test = [&](int k, const Cand<R, DIM> &c) {
     if ((R)me.r2(c) < rv2 && k != i) {
       buf = make_int4(buf.y, buf.z, buf.w, k - pb);
@@ -1931,14 +1927,14 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     for (; k + 8 <= k1; k += 8) {
       Cand<R, DIM> c[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) c[u].load(S, k + u);
+      for (int u = 0; u < 8; ++u) c[u].load(G, k + u);
 #pragma unroll
       for (int u = 0; u < 8; ++u) test(k + u, c[u]);
     }
     if (k + 4 <= k1) {
       Cand<R, DIM> c[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) c[u].load(S, k + u);
+      for (int u = 0; u < 4; ++u) c[u].load(G, k + u);
 #pragma unroll
       for (int u = 0; u < 4; ++u) test(k + u, c[u]);
       k += 4;
@@ -1946,7 +1942,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
 #pragma unroll 1
     for (; k < k1; ++k) {
       Cand<R, DIM> c;
-      c.load(S, k);
+      c.load(G, k);
       test(k, c);
     }
   });
@@ -3540,8 +3536,7 @@ class DeviceImpl final : public Device {
                                                     stage_id_.p, rank_.p, sorted_cell_.p, nc_,
                                                     gate, carry ? newpos_.p : nullptr,
                                                     gate ? 0 : 1);
-    if (gate)
-      k_copy_back<R><<<nblk_, block_, 0, stream_>>>(table(), T, S, nc_, gate, any);
+    // (gated: the list build that follows copies the rebuilt simulations back)
     HIPCHK(hipGetLastError());
     if (carry) {
       // A stand-alone neighbour search: the fields computed before it (density,
@@ -3796,8 +3791,8 @@ class DeviceImpl final : public Device {
         constexpr bool WALL = decltype(wallc)::value;
         // with colloids, the launch also sets the step counter and m(t)
         k_verlet_build<DIM, WALL><<<nblk_, block_, 0, stream_>>>(
-            table(), parts(cur_), cs_.p, vlist(), vctl(), ncoll_ ? steps_.p : nullptr, mag_.p,
-            step);
+            table(), parts(cur_), parts(cur_ ^ 1), nc_, any_.p, cs_.p, vlist(), vctl(),
+            ncoll_ ? steps_.p : nullptr, mag_.p, step);
       });
       // (the density pass that follows turns overflows into next step's rebuilds)
       HIPCHK(hipGetLastError());
