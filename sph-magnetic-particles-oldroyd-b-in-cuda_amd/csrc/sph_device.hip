@@ -1900,24 +1900,29 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
   unsigned sw[4] = {0u, 0u, 0u, 0u};
   uint4 *__restrict__ sout = slots ? VL.slot + i : nullptr;
   // The cut-off test of the density pass (float positions in the FP32 form),
-  // eight candidates' loads in flight at a time.  Entries are indices within
-  // the simulation (k - pbeg), the offsets the passes' buffer loads take.
+  // eight candidates' loads in flight at a time, tested into a hit mask; only
+  // the hits are appended (a wave runs the append as often as its lane with
+  // the most hits in the group, not once per candidate).  Entries are indices
+  // within the simulation (k - pbeg), the offsets the passes' buffer loads
+  // take.  The slots of the current group of eight entries are a 128-bit shift
+  // register: each hit enters at the top, so after eight the oldest is in the
+  // low half of sw[0].
   CandSelf<R, DIM> me;
   me.init(G, i, P);
-  auto test = [&](int k, const Cand<R, DIM> &c) {
-    if ((R)me.r2(c) < rv2 && k != i) {
-      buf = make_int4(buf.y, buf.z, buf.w, k - pb);
-      if (staged) {
-        const int e = cnt & 7;
-        sw[e >> 1] |= slot_of(k) << (16 * (e & 1));
-      }
-      cnt++;
-      if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
-      if (staged && (cnt & 7) == 0) {
-        if (cnt <= cap) sout[(size_t)((cnt >> 3) - 1) * N] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
-        sw[0] = sw[1] = sw[2] = sw[3] = 0u;
-      }
-    }
+  auto hit = [&](int k, const Cand<R, DIM> &c) { return (R)me.r2(c) < rv2 && k != i; };
+  auto push_slot = [&](unsigned v) {
+    sw[0] = __builtin_amdgcn_alignbit(sw[1], sw[0], 16);
+    sw[1] = __builtin_amdgcn_alignbit(sw[2], sw[1], 16);
+    sw[2] = __builtin_amdgcn_alignbit(sw[3], sw[2], 16);
+    sw[3] = __builtin_amdgcn_alignbit(v, sw[3], 16);
+  };
+  auto append = [&](int k) {
+    buf = make_int4(buf.y, buf.z, buf.w, k - pb);
+    if (staged) push_slot(slot_of(k));
+    cnt++;
+    if ((cnt & 3) == 0 && cnt <= cap) out[(size_t)((cnt >> 2) - 1) * N] = buf;
+    if (staged && (cnt & 7) == 0 && cnt <= cap)
+      sout[(size_t)((cnt >> 3) - 1) * N] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
   };
   for_each_reach<DIM, WALL>(P, cs, xi, yi, zi, cx, cy, cz, rv, [&](int k0, int k1, double sx,
                                                                     double sy, double sz) {
@@ -1928,22 +1933,36 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
       Cand<R, DIM> c[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) c[u].load(G, k + u);
+      unsigned h = 0u;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) test(k + u, c[u]);
+      for (int u = 0; u < 8; ++u) h |= (hit(k + u, c[u]) ? 1u : 0u) << u;
+#pragma unroll 1
+      while (h) {  // the hits in index order
+        const int u = __builtin_ctz(h);
+        h &= h - 1u;
+        append(k + u);
+      }
     }
     if (k + 4 <= k1) {
       Cand<R, DIM> c[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) c[u].load(G, k + u);
+      unsigned h = 0u;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) test(k + u, c[u]);
+      for (int u = 0; u < 4; ++u) h |= (hit(k + u, c[u]) ? 1u : 0u) << u;
+#pragma unroll 1
+      while (h) {
+        const int u = __builtin_ctz(h);
+        h &= h - 1u;
+        append(k + u);
+      }
       k += 4;
     }
 #pragma unroll 1
     for (; k < k1; ++k) {
       Cand<R, DIM> c;
       c.load(G, k);
-      test(k, c);
+      if (hit(k, c)) append(k);
     }
   });
   const int tail = cnt & 3;
@@ -1951,8 +1970,11 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
     for (int q = tail; q < 4; ++q) buf = make_int4(buf.y, buf.z, buf.w, 0);
     out[(size_t)(cnt >> 2) * N] = buf;
   }
-  if (staged && (cnt & 7) && cnt < cap)
+  if (staged && (cnt & 7) && cnt < cap) {
+    // the group's entries down to the low slots, zeros above
+    for (int q = cnt & 7; q < 8; ++q) push_slot(0u);
     sout[(size_t)(cnt >> 3) * N] = make_uint4(sw[0], sw[1], sw[2], sw[3]);
+  }
   VL.count[i] = cnt;
   // the displacement checks compare float positions with this reference
   V.ref[i] = make_float4((float)xi, (float)yi, (float)zi, 0.0f);
