@@ -569,10 +569,15 @@ constexpr int kWinTile = 256;                 // particles per window tile
 constexpr int kWinRanges = 24;                // merged index ranges per window
 constexpr int kWinInts = 2 + 2 * kWinRanges;  // [ranges | -1, positions, (first, count)...]
 constexpr int kStageCap = 3584;               // positions per window (3 x 4 B each in LDS)
-// FP64 form: 3 x 8 B each in LDS; two workgroups of the density pass per CU
+// FP64 form, 3D: 3 x 8 B each in LDS; two workgroups of the density pass per
+// CU.  2D (staged only on request): 2 x 8 B each in the 12 KB that the
+// unstaged walk's hand-over takes anyway, so the pass keeps its occupancy.
 constexpr int kStageCap64 = 3376;
+constexpr int kStageCap64_2D = 768;
 template <class R>
-constexpr int kStageCapOf = sizeof(R) == 4 ? kStageCap : kStageCap64;
+constexpr int stage_cap_of(int dim) {
+  return sizeof(R) == 4 ? kStageCap : dim == 3 ? kStageCap64 : kStageCap64_2D;
+}
 constexpr unsigned kNoSlot = 0xFFFFu;
 
 // The window of a tile whose particles occupy cells c0 .. c1 (the simulation's
@@ -1511,14 +1516,15 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     const int nwin = gw ? gw[0] : -1;
     // the LDS of the staged walk's window or of the unstaged walk's position
     // hand-over (a workgroup takes one of the two walks)
-    __shared__ __attribute__((aligned(16))) double lds64[3 * kStageCap64];
+    constexpr int kCap = DIM == 3 ? kStageCap64 : kStageCap64_2D;
+    __shared__ __attribute__((aligned(16))) double lds64[DIM * kCap];
     if (nwin >= 0) {
       // FP64 form, staged walk (as the FP32 one above): the window's positions
       // (the doubles of the packed position copy) are copied into LDS, then
       // each entry's position is read from its slot, entries of the second
       // ring of cells (kNoSlot) from memory.  The same positions, tests and
       // sums in the same order as the unstaged walk below: the same densities.
-      double *wxd = lds64, *wyd = lds64 + kStageCap64, *wzd = lds64 + 2 * kStageCap64;
+      double *wxd = lds64, *wyd = lds64 + kCap, *wzd = lds64 + (DIM == 3 ? 2 * kCap : 0);
       __shared__ int wr[kWinInts];
       if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
       __syncthreads();
@@ -1564,7 +1570,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-          const unsigned sc = min(sk, (unsigned)(kStageCap64 - 1));
+          const unsigned sc = min(sk, (unsigned)(kCap - 1));
           px[u] = wxd[sc];
           py[u] = wyd[sc];
           pz[u] = DIM == 3 ? wzd[sc] : 0.0;
@@ -3308,9 +3314,9 @@ class DeviceImpl final : public Device {
     int stage_mode = 1;
     if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_mode = std::atoi(e);
     stage_ = verlet_ok_ && stage_mode != 0 && (P_.dim == 3 || stage_mode == 2);
-    stage_cap_ = kStageCapOf<R>;
+    stage_cap_ = stage_cap_of<R>(P_.dim);
     if (const char *e = std::getenv("SPH_STAGE_CAP"))
-      stage_cap_ = std::min(kStageCapOf<R>, std::max(0, std::atoi(e)));
+      stage_cap_ = std::min(stage_cap_of<R>(P_.dim), std::max(0, std::atoi(e)));
     if (verlet_ok_) {
       HIPCHK(vl_idx_.alloc((size_t)(vcap_ / 4) * N_));
       if (stage_) {
@@ -4311,7 +4317,7 @@ class DeviceImpl final : public Device {
   DevBuf<int> vl_win_;     // VList::win
   DevBuf<int> wbase_;      // VList::wbase
   bool stage_ = false;
-  int stage_cap_ = kStageCapOf<R>;
+  int stage_cap_ = 0;
   DevBuf<unsigned> vl_mask_;
   DevBuf<int> vl_count_, vrebuild_, voverflow_;
   DevBuf<int> any_;  // [1] batch rebuild flag (VerletCtl::any)
