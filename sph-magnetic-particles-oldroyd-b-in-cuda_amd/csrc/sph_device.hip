@@ -180,7 +180,7 @@ struct VList {
   uint4 *slot;
   int *win;
   const int *wbase;
-  int stage_cap;    // positions per window (<= kStageCap)
+  int stage_cap;    // positions per window (<= the LDS window of the form)
   int cap;
   float skin_frac;  // skin / rcut
 };
@@ -569,6 +569,7 @@ constexpr int kWinTile = 256;                 // particles per window tile
 constexpr int kWinRanges = 24;                // merged index ranges per window
 constexpr int kWinInts = 2 + 2 * kWinRanges;  // [ranges | -1, positions, (first, count)...]
 constexpr int kStageCap = 3584;               // positions per window (3 x 4 B each in LDS)
+constexpr int kStageCap2D = 1792;             // 2D: 2 x 4 B each (a 2D window holds ~800)
 // FP64 form, 3D: 3 x 8 B each in LDS; two workgroups of the density pass per
 // CU.  2D (staged only on request): 2 x 8 B each in the 12 KB that the
 // unstaged walk's hand-over takes anyway, so the pass keeps its occupancy.
@@ -576,7 +577,8 @@ constexpr int kStageCap64 = 3376;
 constexpr int kStageCap64_2D = 768;
 template <class R>
 constexpr int stage_cap_of(int dim) {
-  return sizeof(R) == 4 ? kStageCap : dim == 3 ? kStageCap64 : kStageCap64_2D;
+  return sizeof(R) == 4 ? (dim == 3 ? kStageCap : kStageCap2D)
+                        : (dim == 3 ? kStageCap64 : kStageCap64_2D);
 }
 constexpr unsigned kNoSlot = 0xFFFFu;
 
@@ -1408,7 +1410,8 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
         // its slot; entries of the second ring of cells (kNoSlot) take their
         // index from the list and their position from memory.  The same
         // positions in the same order as the unstaged walk: the same sums.
-        __shared__ unsigned wx[kStageCap], wy[kStageCap], wz[DIM == 3 ? kStageCap : 1];
+        constexpr int kCap = DIM == 3 ? kStageCap : kStageCap2D;
+        __shared__ unsigned wx[kCap], wy[kCap], wz[DIM == 3 ? kCap : 1];
         __shared__ int wr[kWinInts];
         if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
         __syncthreads();
@@ -1454,7 +1457,7 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-            const unsigned sc = min(sk, (unsigned)(kStageCap - 1));
+            const unsigned sc = min(sk, (unsigned)(kCap - 1));
             c[u] = make_uint4(wx[sc], wy[sc], DIM == 3 ? wz[sc] : 0u, 1u);
             far |= sk == kNoSlot && g + u < vc;
           }
@@ -3305,15 +3308,15 @@ class DeviceImpl final : public Device {
     vcap_ = ((int)(1.5 * vmean) + 64 + 7) / 8 * 8;
     if (const char *e = std::getenv("SPH_VERLET_CAPACITY"))
       vcap_ = std::max(8, (std::atoi(e) + 7) / 8 * 8);
-    // Staged density walk, 3D (SPH_DENSITY_STAGE=0 turns it off, =2 turns
-    // it on in 2D too; SPH_STAGE_CAP lowers the positions a window may hold,
-    // so that more tiles take the unstaged walk; results are the same either
-    // way).  In 2D (about 30 entries per particle) the window copy and the
-    // slots of the frequent rebuilds of a batch cost more than the L1 gathers
-    // they replace: a 55-simulation sweep batch stepped 10 % slower.
+    // Staged density walk: 3D, and 2D in the FP32 form (SPH_DENSITY_STAGE=0
+    // turns it off, =2 turns it on for the FP64 form in 2D too; SPH_STAGE_CAP
+    // lowers the positions a window may hold, so that more tiles take the
+    // unstaged walk; results are the same either way).  The 256-simulation 2D
+    // sweep batch steps 1-2 % faster staged.
     int stage_mode = 1;
     if (const char *e = std::getenv("SPH_DENSITY_STAGE")) stage_mode = std::atoi(e);
-    stage_ = verlet_ok_ && stage_mode != 0 && (P_.dim == 3 || stage_mode == 2);
+    stage_ = verlet_ok_ && stage_mode != 0 &&
+             (P_.dim == 3 || sizeof(R) == 4 || stage_mode == 2);
     stage_cap_ = stage_cap_of<R>(P_.dim);
     if (const char *e = std::getenv("SPH_STAGE_CAP"))
       stage_cap_ = std::min(stage_cap_of<R>(P_.dim), std::max(0, std::atoi(e)));
