@@ -387,11 +387,10 @@ def test_observables(case, tmp_path):
 @pytest.mark.parametrize("case", ["colloids3d", "sine3d", "couette2d"])
 def test_staged_density_walk_is_the_unstaged_one(case, precision, tmp_path, monkeypatch):
     """The density walk reads neighbour positions from a per-tile LDS
-    window (SPH_DENSITY_STAGE=2: in 2D too; =0 reads them from memory;
-    SPH_STAGE_CAP=400 leaves the tiles with larger windows to the unstaged
-    walk).  The same
-    positions in the same order: bitwise the same trajectory, with list
-    rebuilds (a skin of 0.01 rcut) inside the run."""
+    window (SPH_DENSITY_STAGE=2: in 3D, and in 2D for both forms; =0 reads
+    them from memory; SPH_STAGE_CAP=400 leaves the tiles with larger windows
+    to the unstaged walk).  The same positions in the same order: bitwise the
+    same trajectory, with list rebuilds (a skin of 0.01 rcut) inside the run."""
     monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
     cfg = {"couette2d": lambda: helpers.cfg_couette(n=48, Vwall_bottom=-2.0, Vwall_top=2.0),
            "colloids3d": lambda: helpers.cfg_colloids_3d(n=24),
