@@ -382,7 +382,11 @@ __device__ __forceinline__ int xcd_tile() {
 #define SPH_PAIRS_WAVES_F32 1
 #endif
 #ifndef SPH_PAIRS_WAVES_F64
-#define SPH_PAIRS_WAVES_F64 1
+#define SPH_PAIRS_WAVES_F64 3
+#endif
+// Record loads of the pair walk straight into LDS (0: through registers).
+#ifndef SPH_PAIRS_LDS_LOADS
+#define SPH_PAIRS_LDS_LOADS 1
 #endif
 // and for the density pass
 #ifndef SPH_DENSITY_WAVES
@@ -1217,6 +1221,15 @@ __device__ __forceinline__ int quad_bcast(int x) {
 template <int C>
 __device__ __forceinline__ int quad_pick(int x) {
   return __builtin_amdgcn_update_dpp(0, x, C, 0xF, 0xF, false);
+}
+
+// A 16-byte buffer load per lane whose destination is LDS: lane L's chunk lands
+// at dst + 16 L (dst wave-uniform).  Nothing waits for it: the caller waits
+// (vmcnt) before reading the chunks.
+typedef unsigned lds_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load_to_lds(__amdgpu_buffer_rsrc_t rs, lds_u4 *dst, int voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)dst, 16,
+                                           voff, 0, 0, 0);
 }
 
 // The records of one simulation as a buffer resource (a scalar descriptor:
@@ -2366,8 +2379,28 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     constexpr int kLoads = kChunks;                    // group instructions per entry
     constexpr int kPer = kChunks / kG;                 // instructions per record
     constexpr int kSlot = kChunks + 1;  // padded LDS slots: conflict-free rows
-    __shared__ u4 stage[kBlock / 64][64][kSlot];
-    u4(&st)[64][kSlot] = stage[threadIdx.x >> 6];
+    // The loader runs kAhead entries ahead of the evaluation (below).  In the
+    // FP64 form the group loads go straight into LDS (buffer loads with the
+    // LDS as destination: instruction k of a wave writes its 64 lanes' chunks
+    // to row k of the wave's buffer, lane after lane), so no registers hold
+    // the chunks in flight (64 fewer: three wavefronts per SIMD instead of
+    // two, the FP64 step 5 % faster) and nothing is written to LDS by the wave
+    // itself.  The compiler does not wait for such loads before reading the
+    // LDS they write, so the walk waits itself: for every vector memory load
+    // before it reads an entry (vmcnt 0), and for its LDS reads before it
+    // reloads the buffer (lgkmcnt 0).  (The FP32 form keeps its loads in
+    // registers: its 32 freed registers did not buy a wavefront, and the
+    // waits made the pass 5 % slower, 14 % at five wavefronts per SIMD.)
+    constexpr int kAhead = kChunks <= 2 ? 2 : 1;
+    constexpr bool kDma = SPH_PAIRS_LDS_LOADS && kAhead == 1 && sizeof(R) == 8;
+    constexpr int kRow = 64 + 1;  // (a padded row: the four lanes of a group read distinct banks)
+    // (the LDS of the walk not taken shrinks to one element)
+    constexpr int kStW = kDma ? 1 : kBlock / 64, kStL = kDma ? 1 : 64, kStS = kDma ? 1 : kSlot;
+    constexpr int kDbW = kDma ? kBlock / 64 : 1, kDbL = kDma ? kLoads : 1, kDbR = kDma ? kRow : 1;
+    __shared__ u4 stage[kStW][kStL][kStS];
+    u4(&st)[kStL][kStS] = stage[kDma ? 0 : threadIdx.x >> 6];
+    __shared__ u4 dmab[kDbW][kDbL][kDbR];
+    u4(&db)[kDbL][kDbR] = dmab[kDma ? threadIdx.x >> 6 : 0];
     const int lane = threadIdx.x & 63, qb = lane & (kG - 1), qa = lane & ~(kG - 1);
     const int vc = vok ? VL.count[i] : 0;
     int vmax = vc;
@@ -2415,7 +2448,6 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     // loads are in flight for kAhead entries of work.  Two ahead for the
     // two-chunk record (2D FP32: its batched sweep steps 2 % faster); the larger
     // records measured no faster two ahead, for more registers.
-    constexpr int kAhead = kChunks <= 2 ? 2 : 1;
     // Entries per trip, and chunk buffers (each entry's buffer is fixed within
     // a trip; only kAhead + 1 of the four are live at a time).  Four entries
     // per trip start every trip at a multiple of four, so that each entry's
@@ -2444,6 +2476,46 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
       const int j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
       quad_load((m & 1u) ? (unsigned)j * (unsigned)sizeof(Rec) : oob, v);
     };
+    if constexpr (kDma) {
+      constexpr int kVm0 = 0x0F70, kLgkm0 = 0xC07F;  // s_waitcnt vmcnt(0) / lgkmcnt(0)
+      // lane qa + qb's chunk c sits in row qb kPer + c / kG at lane qa + c % kG
+      const u4 *mrow = &db[qb * kPer][qa];
+      auto dma_at = [&](int k) {
+        const int c = k & 3;  // (known at compile time, see kTrip)
+        const int j = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
+        const int mine = (int)((m & 1u) ? (unsigned)j * (unsigned)sizeof(Rec) : oob);
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+          const int o = src(u) == 0 ? quad_bcast<0>(mine)
+                      : src(u) == 1 ? quad_bcast<1>(mine)
+                      : src(u) == 2 ? quad_bcast<2>(mine) : quad_bcast<3>(mine);
+          load_to_lds(rb.rs, &db[u][0], o + 16 * chk(u));
+        }
+      };
+      // hit: the hit bit of the entry whose loads are in flight
+      unsigned hitb = m & 1u;
+      dma_at(0);
+      auto entry_lds = [&](int e) {
+        const bool hit = hitb;
+        __builtin_amdgcn_s_waitcnt(kVm0);  // entry e has landed
+        u4 mine[kRecVecs<R, DIM>];
+#pragma unroll
+        for (int c = 0; c < kRecVecs<R, DIM>; ++c) mine[c] = mrow[(c / kG) * kRow + c % kG];
+        __builtin_amdgcn_s_waitcnt(kLgkm0);  // read before the buffer is reloaded
+        advance(e + 1);
+        hitb = m & 1u;
+        dma_at(e + 1);
+        if (hit) pair_body<R, DIM, WALL, DO_GV, DO_F>(K, unpack_chunks<R, DIM>(mine), cv, me, acc);
+      };
+#pragma unroll 1
+      for (int e = 0; e < vmax; e += kTrip) {
+#pragma unroll
+        for (int t = 0; t < kTrip; ++t) entry_lds(e + t);
+      }
+      // (the loads of the entry past the walk are waited for: nothing of this
+      // wave writes the buffer after it leaves)
+      __builtin_amdgcn_s_waitcnt(kVm0);
+    } else {
     // pend: hit bits of the entries loaded but not yet evaluated (bit 0: the
     // next to evaluate)
     u4 buf[kTrip][kLoads];
@@ -2480,6 +2552,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_
     for (int e = 0; e < vmax; e += kTrip) {
 #pragma unroll
       for (int t = 0; t < kTrip; ++t) entry(e + t, buf[t], buf[(t + kAhead) % kTrip]);
+    }
     }
     if (!live) return;
     if (vok) {
