@@ -377,12 +377,17 @@ __device__ __forceinline__ int xcd_tile() {
 }
 
 // Minimum resident wavefronts per SIMD requested for the fused pair kernel
-// (caps its registers), per precision form.
+// (caps its registers), per precision form; the FP64 form with walls at three
+// (1M-particle walls + colloids step 1.49 -> 1.46 ms), the periodic FP64 form
+// uncapped (its 64^3 step is 0.339 ms uncapped, 0.357 ms at three).
 #ifndef SPH_PAIRS_WAVES_F32
 #define SPH_PAIRS_WAVES_F32 1
 #endif
 #ifndef SPH_PAIRS_WAVES_F64
 #define SPH_PAIRS_WAVES_F64 3
+#endif
+#ifndef SPH_PAIRS_WAVES_F64_PERIODIC
+#define SPH_PAIRS_WAVES_F64_PERIODIC 1
 #endif
 // Record loads of the pair walk straight into LDS (0: through registers).
 #ifndef SPH_PAIRS_LDS_LOADS
@@ -2310,7 +2315,8 @@ __device__ __forceinline__ void pair_finish(const sph_params_t &P, const Particl
 // CST: the simulations' colloid states are staged in LDS (every simulation of the
 // batch has at most kCollStage colloids).
 template <class R, int DIM, bool WALL, bool DO_GV, bool DO_F, bool CST>
-__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32 : SPH_PAIRS_WAVES_F64) void k_pairs(
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? SPH_PAIRS_WAVES_F32
+                                    : WALL ? SPH_PAIRS_WAVES_F64 : SPH_PAIRS_WAVES_F64_PERIODIC) void k_pairs(
     SimTable T, Particles<R> S, Derived<R> D, Bodies Ball,
     const PairRec<R, DIM> *__restrict__ rec,
     NList L, int store_gv, int write_dcdt, int with_ext, int kick, VList VL) {
