@@ -414,3 +414,30 @@ def test_staged_density_walk_is_the_unstaged_one(case, precision, tmp_path, monk
     for ref in outs[1:]:
         for n, a, b in zip(names, outs[0], ref):
             np.testing.assert_array_equal(a, b, err_msg=n)
+
+
+@pytest.mark.parametrize("precision", [sph_system.FP32, sph_system.FP64], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("case", ["colloids3d", "sine3d"])
+def test_tile_order_leaves_the_trajectory_alone(case, precision, tmp_path, monkeypatch):
+    """A single 3D simulation walks each XCD's slab of tiles in y bands
+    (SPH_TILE_ORDER=k bands, 0: layer after layer); tiles are independent, so
+    every order gives bitwise the same trajectory, list rebuilds (a skin of
+    0.01 rcut) included."""
+    monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
+    cfg = {"colloids3d": lambda: helpers.cfg_colloids_3d(n=24),
+           "sine3d": lambda: helpers.cfg_sine(n=24)}[case]()
+    names = ("x", "y", "z", "vx", "vy", "vz", "dens", "fx", "fy", "fz", "cxy", "czz")
+    outs = []
+    for bands in ("0", "4", "3"):
+        monkeypatch.setenv("SPH_TILE_ORDER", bands)
+        s = helpers.make_system(cfg, str(tmp_path), precision, noise=1e-2,
+                                name=f"o{len(outs)}")
+        helpers.perturb(s)
+        s.prepare()
+        s.step(12)
+        s.synchronize()
+        outs.append([s.get(n) for n in names])
+        s.close()
+    for ref in outs[1:]:
+        for n, a, b in zip(names, outs[0], ref):
+            np.testing.assert_array_equal(a, b, err_msg=n)
