@@ -114,6 +114,32 @@ def test_launch_shape_does_not_change_results(tmp_path):
                                               err_msg=f"{block} {xcd} sim {k} {n}")
 
 
+def test_launch_shape_of_one_3d_simulation(tmp_path, monkeypatch):
+    """A single 3D simulation (a batch of one) under every launch shape, the
+    shape changed mid-run: its tile order (y bands of each XCD's share), list
+    windows and rebuilds (skin 0.01 rcut) follow the shape, the trajectory is
+    bitwise the same."""
+    monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
+    cfg = helpers.cfg_colloids_3d(n=24)
+
+    def run(name, shapes):
+        b = sph_system.Batch(sph_system.FP32)
+        b.add(helpers.make_system(cfg, str(tmp_path), sph_system.FP32, noise=1e-2, name=name))
+        b.set_launch(*shapes[0])
+        b.prepare()
+        b.step(4)
+        b.set_launch(*shapes[1])
+        b.step(4)
+        return {n: b.get(0, n) for n in ("x", "y", "z", "vx", "vz", "cxy", "fx", "dens")}
+
+    ref = run("r", ((256, True), (256, True)))
+    for shapes in (((64, True), (128, False)), ((128, True), (64, True)),
+                   ((256, False), (256, True))):
+        out = run(f"l{shapes[0][0]}{shapes[1][0]}", shapes)
+        for n in ref:
+            np.testing.assert_array_equal(out[n], ref[n], err_msg=f"{shapes} {n}")
+
+
 def test_autotune_keeps_the_trajectory(tmp_path):
     singles = _systems("2d", str(tmp_path), sph_system.FP64, "s")
     b = sph_system.Batch(sph_system.FP64)
