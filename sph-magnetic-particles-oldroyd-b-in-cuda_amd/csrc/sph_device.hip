@@ -398,40 +398,49 @@ __device__ __forceinline__ int xcd_tile() {
 #define SPH_DENSITY_WAVES 1
 #endif
 
-// kernel_W (src/kernel_W.cu:13-59) as selects: cw*((a^5 - 6 b^5) + 15 c^5).
+// max(x, 0) of an x <= 1: fmin(fmax(x, 0), 1) becomes the clamp output
+// modifier of the instruction that forms x (no instruction of its own).
+template <class R>
+__device__ __forceinline__ R clamp01(R x) { return fmin(fmax(x, R(0)), R(1)); }
+
+// The three spline terms below use, for 0 <= q3 < 3, a = 3 - q3, h = b / 2 =
+// max(1 - q3 / 2, 0) and c = max(1 - q3, 0), both clamps free (clamp01).  A term
+// past its range is an exact 0, and b^5 = 32 h^5, b^4 = 16 h^4 hold exactly
+// (1 - q3 / 2 rounds to half of 2 - q3 rounded, products of h are normal
+// numbers, and powers of two scale without rounding), so every result is
+// bitwise the one of the piecewise form with b = 2 - q3.
+
+// kernel_W (src/kernel_W.cu:13-59): cw*((a^5 - 6 b^5) + 15 c^5), the b term
+// where q3 < 2 and the c term where q3 < 1, in the reference's order (each
+// product rounded before its sum: no fused multiply-adds).
 template <class R>
 __device__ __forceinline__ R sph_W(R r, R rcut_inv, R cw) {
-  R q3 = r * rcut_inv * R(3);
-  R a = R(3) - q3, a2 = a * a, t = a2 * a2 * a;
-  R b = R(2) - q3, b2 = b * b, b5 = R(6) * (b2 * b2 * b);
-  R c = R(1) - q3, c2 = c * c, c5 = R(15) * (c2 * c2 * c);
-  if (q3 < R(2)) t = t - b5;
-  if (q3 < R(1)) t = t + c5;
-  return cw * t;
+#pragma clang fp contract(off)
+  const R q3 = r * rcut_inv * R(3);
+  const R a = R(3) - q3, a2 = a * a;
+  const R h = clamp01(fma(R(-0.5), q3, R(1))), h2 = h * h;
+  const R c = clamp01(R(1) - q3), c2 = c * c;
+  return cw * ((a2 * a2 * a - R(192) * (h2 * h2 * h)) + R(15) * (c2 * c2 * c));
 }
 
 // The spline polynomial of kernel_W without cw, from q3 = 3 r / rcut, so that a
-// sum of W is cw times a sum of these; branch-free: a^5 - 6 b^5 + 15 c^5 with
-// a = 3 - q3, b = max(2 - q3, 0), c = max(1 - q3, 0) (a term past its range is
-// an exact 0).
+// sum of W is cw times a sum of these: a^5 - 6 b^5 + 15 c^5.
 template <class R>
 __device__ __forceinline__ R sph_W_poly(R q3) {
   const R a = R(3) - q3, a2 = a * a;
-  const R b = fmax(R(2) - q3, R(0)), b2 = b * b;
-  const R c = fmax(R(1) - q3, R(0)), c2 = c * c;
-  return fma(R(15), c2 * c2 * c, fma(R(-6), b2 * b2 * b, a2 * a2 * a));
+  const R h = clamp01(fma(R(-0.5), q3, R(1))), h2 = h * h;
+  const R c = clamp01(R(1) - q3), c2 = c * c;
+  return fma(R(15), c2 * c2 * c, fma(R(-192), h2 * h2 * h, a2 * a2 * a));
 }
 
 // kernel_gradW (src/kernel_gradW.cu:13-53) is c_gradw*((a^4 - 6 b^4) + 15 c^4)
-// piecewise in q3 = 3 r / rcut.  Its polynomial without c_gradw (0 <= q3 < 3),
-// branch-free: a^4 - 6 b^4 + 15 c^4 with a = 3 - q3, b = max(2 - q3, 0),
-// c = max(1 - q3, 0) - a term past its range is an exact 0.
+// piecewise in q3 = 3 r / rcut.  Its polynomial without c_gradw (0 <= q3 < 3).
 template <class R>
 __device__ __forceinline__ R sph_gradW_poly(R q3) {
   const R a = R(3) - q3, a2 = a * a;
-  const R b = fmax(R(2) - q3, R(0)), b2 = b * b;
-  const R c = fmax(R(1) - q3, R(0)), c2 = c * c;
-  return fma(R(15), c2 * c2, fma(R(-6), b2 * b2, a2 * a2));
+  const R h = clamp01(fma(R(-0.5), q3, R(1))), h2 = h * h;
+  const R c = clamp01(R(1) - q3), c2 = c * c;
+  return fma(R(15), c2 * c2, fma(R(-96), h2 * h2, a2 * a2));
 }
 
 // Unclamped cell coordinates of a particle, as the pair kernels of the
@@ -567,8 +576,8 @@ __device__ __forceinline__ void for_each_reach(const sph_params_t &P, const int 
 // of the passes lies inside one) gets a window: the sorted-index ranges of the
 // cells within one cell per axis of the cells its particles occupy (periodic
 // images folded in, rows beyond a wall dropped), merged.  Every list entry also
-// gets its slot in the window, or kNoSlot for the few entries that a build's
-// reach adds from the second ring of cells.  The density pass copies the
+// gets its slot in the window, or the far slot (kFarSlot) for the few entries
+// that a build's reach adds from the second ring of cells.  The density pass copies the
 // window's positions (FP32 form: fixed point; FP64 form: doubles) into LDS
 // once per workgroup and reads each entry's position from there: the gathers
 // no longer go through the vector L1
@@ -589,7 +598,14 @@ constexpr int stage_cap_of(int dim) {
   return sizeof(R) == 4 ? (dim == 3 ? kStageCap : kStageCap2D)
                         : (dim == 3 ? kStageCap64 : kStageCap64_2D);
 }
-constexpr unsigned kNoSlot = 0xFFFFu;
+// The slot of a list entry outside its tile's window: the LDS element just past
+// the window's capacity, which the walk reads like any other slot (and then
+// replaces with the position from memory), so that slots need no clamp.  The
+// walk's LDS arrays hold kStageLds(cap) elements (a multiple of 64, so that the
+// coordinate arrays stay 256 B-aligned for paired LDS reads).
+template <class R, int DIM>
+constexpr unsigned kFarSlot = (unsigned)stage_cap_of<R>(DIM);
+constexpr int kStageLds(int cap) { return (cap + 64) / 64 * 64; }
 
 // The window of a tile whose particles occupy cells c0 .. c1 (the simulation's
 // cell indices, sorted order), built by the 64 lanes of one wavefront: w[0] =
@@ -1427,11 +1443,14 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
       if (nwin >= 0) {
         // Staged walk: the window's positions are copied into LDS (coalesced
         // runs of the sorted order), then each entry's position is read from
-        // its slot; entries of the second ring of cells (kNoSlot) take their
+        // its slot; entries of the second ring of cells (the far slot) take their
         // index from the list and their position from memory.  The same
         // positions in the same order as the unstaged walk: the same sums.
         constexpr int kCap = DIM == 3 ? kStageCap : kStageCap2D;
-        __shared__ unsigned wx[kCap], wy[kCap], wz[DIM == 3 ? kCap : 1];
+        constexpr int kLds = kStageLds(kCap);
+        constexpr unsigned kFar = kFarSlot<R, DIM>;
+        static_assert(kFar == (unsigned)kCap && kFar < (unsigned)kLds, "far slot in LDS");
+        __shared__ unsigned wx[kLds], wy[kLds], wz[DIM == 3 ? kLds : 1];
         __shared__ int wr[kWinInts];
         if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
         __syncthreads();
@@ -1469,23 +1488,23 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
         for (int g = 0; g < vc; g += 8) {
           const uint4 sn = slot_group((g >> 3) + 1);
           const unsigned w4[4] = {sg.x, sg.y, sg.z, sg.w};
-          // every slot is read (clamped into the window: entries past the list
-          // hold any slot and are masked below); entries of the second ring of
-          // cells are fetched from memory afterwards
+          // every slot is read (every slot is inside the LDS arrays: window
+          // slots, the far slot, and the 0 that pads a list's last group of
+          // eight, which is masked below); entries of the second ring of cells
+          // are fetched from memory afterwards
           uint4 c[8];
           bool far = false;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-            const unsigned sc = min(sk, (unsigned)(kCap - 1));
-            c[u] = make_uint4(wx[sc], wy[sc], DIM == 3 ? wz[sc] : 0u, 1u);
-            far |= sk == kNoSlot && g + u < vc;
+            c[u] = make_uint4(wx[sk], wy[sk], DIM == 3 ? wz[sk] : 0u, 1u);
+            far |= sk == kFar;
           }
           if (far) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
               const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-              if (sk == kNoSlot && g + u < vc) {
+              if (sk == kFar && g + u < vc) {
                 const int4 q4 = vl[(size_t)((g + u) >> 2) * N];
                 const int cq = (g + u) & 3;
                 const int j = cq == 0 ? q4.x : cq == 1 ? q4.y : cq == 2 ? q4.z : q4.w;
@@ -1540,14 +1559,17 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
     // the LDS of the staged walk's window or of the unstaged walk's position
     // hand-over (a workgroup takes one of the two walks)
     constexpr int kCap = DIM == 3 ? kStageCap64 : kStageCap64_2D;
-    __shared__ __attribute__((aligned(16))) double lds64[DIM * kCap];
+    constexpr int kLds = kCap + 1;  // (the far slot; 2 x 3 x kLds doubles fit one CU)
+    constexpr unsigned kFar = kFarSlot<R, DIM>;
+    static_assert(kFar == (unsigned)kCap && kFar < (unsigned)kLds, "far slot in LDS");
+    __shared__ __attribute__((aligned(16))) double lds64[DIM * kLds];
     if (nwin >= 0) {
       // FP64 form, staged walk (as the FP32 one above): the window's positions
       // (the doubles of the packed position copy) are copied into LDS, then
       // each entry's position is read from its slot, entries of the second
-      // ring of cells (kNoSlot) from memory.  The same positions, tests and
+      // ring of cells (the far slot) from memory.  The same positions, tests and
       // sums in the same order as the unstaged walk below: the same densities.
-      double *wxd = lds64, *wyd = lds64 + kCap, *wzd = lds64 + (DIM == 3 ? 2 * kCap : 0);
+      double *wxd = lds64, *wyd = lds64 + kLds, *wzd = lds64 + (DIM == 3 ? 2 * kLds : 0);
       __shared__ int wr[kWinInts];
       if ((int)threadIdx.x < 2 + 2 * nwin) wr[threadIdx.x] = gw[threadIdx.x];
       __syncthreads();
@@ -1593,17 +1615,16 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-          const unsigned sc = min(sk, (unsigned)(kCap - 1));
-          px[u] = wxd[sc];
-          py[u] = wyd[sc];
-          pz[u] = DIM == 3 ? wzd[sc] : 0.0;
-          far |= sk == kNoSlot && g + u < vc;
+          px[u] = wxd[sk];
+          py[u] = wyd[sk];
+          pz[u] = DIM == 3 ? wzd[sk] : 0.0;
+          far |= sk == kFar;
         }
         if (far) {
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const unsigned sk = (w4[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
-            if (sk == kNoSlot && g + u < vc) {
+            if (sk == kFar && g + u < vc) {
               const int4 q4 = vl[(size_t)((g + u) >> 2) * N];
               const int cq = (g + u) & 3;
               const int j = cq == 0 ? q4.x : cq == 1 ? q4.y : cq == 2 ? q4.z : q4.w;
@@ -1908,7 +1929,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(SimTable T, Particles<R
       if (wwin[2 + 2 * mid] <= k) lo = mid; else hi = mid - 1;
     }
     const int f = wwin[2 + 2 * lo], n = wwin[3 + 2 * lo];
-    if (k < f || k >= f + n) return kNoSlot;
+    if (k < f || k >= f + n) return kFarSlot<R, DIM>;
     wlo = f;
     whi = f + n;
     wsb = woff[lo] - f;
