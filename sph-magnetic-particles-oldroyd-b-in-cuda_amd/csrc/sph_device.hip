@@ -1514,15 +1514,19 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
             }
           }
           // (branch-free: the term of every entry is formed and a miss adds
-          // +0, which leaves the sum bitwise as if the miss were skipped)
+          // +0, which leaves the sum bitwise as if the miss were skipped; the
+          // group's eight hit bits are gathered at constant positions, then
+          // placed in the mask word at once)
+          unsigned m8 = 0u;
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
             const R r2 = (R)me.r2(Cand<R, DIM>{c[u]});
             const R t = w_term(r2);
             const bool hit = g + u < vc && r2 < rcutsq;
             tsum = tsum + (hit ? t : R(0));
-            mw |= (hit ? 1u : 0u) << ((g + u) & 31);
+            m8 |= hit ? 1u << u : 0u;
           }
+          mw |= m8 << (g & 31);
           if (((g + 8) & 31) == 0 || g + 8 >= vc) {
             VL.mask[(size_t)(g >> 5) * N + i] = mw;
             cnt += __builtin_popcount(mw);
