@@ -81,7 +81,7 @@ TRAFFIC_FILE = "profiles/traffic_walls_colloids_1M_fp32.json"
 PAIR_PROFILE = "profiles/pair_passes_walls_colloids_1M_fp32.md"
 # VALU instructions x 4 cycles / SIMD busy time of the FP32 pair passes on the
 # 1M-particle workload (SQ_INSTS_VALU, SQ_BUSY_CYCLES; see PAIR_PROFILE)
-VALU_BUSY = {"k_pairs": 0.63, "k_density": 0.84}
+VALU_BUSY = {"k_pairs": 0.63, "k_density": 0.88}
 
 
 def measured_traffic(kernel, N):
