@@ -383,15 +383,18 @@ def test_observables(case, tmp_path):
     np.testing.assert_allclose(a[~both_nan], b[~both_nan], rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("skin", ["0.01", "0.5"])
 @pytest.mark.parametrize("precision", [sph_system.FP32, sph_system.FP64], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("case", ["colloids3d", "sine3d", "couette2d"])
-def test_staged_density_walk_is_the_unstaged_one(case, precision, tmp_path, monkeypatch):
+def test_staged_density_walk_is_the_unstaged_one(case, precision, skin, tmp_path, monkeypatch):
     """The density walk reads neighbour positions from a per-tile LDS
     window (SPH_DENSITY_STAGE=2: in 3D, and in 2D for both forms; =0 reads
     them from memory; SPH_STAGE_CAP=400 leaves the tiles with larger windows
     to the unstaged walk).  The same positions in the same order: bitwise the
-    same trajectory, with list rebuilds (a skin of 0.01 rcut) inside the run."""
-    monkeypatch.setenv("SPH_VERLET_SKIN", "0.01")
+    same trajectory, with list rebuilds inside the run.  A skin of 0.01 rcut
+    rebuilds every few steps; one of 0.5 rcut lists many entries from the
+    second ring of cells, outside their tile's window (read from memory)."""
+    monkeypatch.setenv("SPH_VERLET_SKIN", skin)
     cfg = {"couette2d": lambda: helpers.cfg_couette(n=48, Vwall_bottom=-2.0, Vwall_top=2.0),
            "colloids3d": lambda: helpers.cfg_colloids_3d(n=24),
            "sine3d": lambda: helpers.cfg_sine(n=24)}[case]()
