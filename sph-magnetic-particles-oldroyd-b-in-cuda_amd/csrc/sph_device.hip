@@ -1639,20 +1639,22 @@ __global__ __launch_bounds__(kBlock, SPH_DENSITY_WAVES) void k_density(SimTable 
             }
           }
         }
+        // (branch-free as in the FP32 form: a miss, or an entry past the list,
+        // adds +0 to the positive d, which leaves it bitwise unchanged)
+        unsigned m8 = 0u;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          if (g + u < vc) {
-            R rs[3];
-            pair_sep<R, DIM, WALL>(bx, xi, yi, zi, px[u], py[u], DIM == 3 ? pz[u] : 0.0, rs);
-            const R r2 = DIM == 3 ? rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2]
-                                  : rs[0] * rs[0] + rs[1] * rs[1];
-            if (r2 < rcutsq) {
-              d = d + sph_W<R>(pair_sqrt(r2), rinv, cw);
-              mw |= 1u << ((g + u) & 31);
-              cnt++;
-            }
-          }
+          R rs[3];
+          pair_sep<R, DIM, WALL>(bx, xi, yi, zi, px[u], py[u], DIM == 3 ? pz[u] : 0.0, rs);
+          const R r2 = DIM == 3 ? rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2]
+                                : rs[0] * rs[0] + rs[1] * rs[1];
+          const R t = sph_W<R>(pair_sqrt(r2), rinv, cw);
+          const bool hit = g + u < vc && r2 < rcutsq;
+          d = d + (hit ? t : R(0));
+          m8 |= hit ? 1u << u : 0u;
         }
+        mw |= m8 << (g & 31);
+        cnt += __builtin_popcount(m8);
         if (((g + 8) & 31) == 0 || g + 8 >= vc) {
           VL.mask[(size_t)(g >> 5) * N + i] = mw;
           mw = 0u;
